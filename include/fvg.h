@@ -1,0 +1,156 @@
+/* fvg — MI355X (gfx950) residual and time step of Fenice's uniform Cartesian grid, behind a C ABI.
+ *
+ * State layout across the boundary: the reference's `Field` (cpp/include/fv/Core.hpp:20-24), i.e. an
+ * array-of-structures of cells, cell (i, j, k) at index (k*ny + j)*nx + i, each cell holding
+ * ncomp doubles [rho, rho*u, rho*v, rho*E(, rho*w)]. ncomp is 5 (the reference's State) or 4 (a 2-D
+ * field without its rho*w slot, the layout of the reference's npz dumps, IO.hpp:153-158). On the device
+ * the state lives as structure-of-arrays FP64 planes; the layout change happens only in upload/download.
+ *
+ * Every entry point returns an fvg_status; the message of the last failure of a context is
+ * fvg_last_error(ctx). The reference raises exceptions instead: std::invalid_argument for bad names and
+ * values (cpp/src/Schemes.cpp:84,93,101, cpp/src/Boundaries.cpp:27), std::logic_error for broken
+ * contracts (cpp/src/Schemes.cpp:398-399), std::runtime_error when a run diverges
+ * (cpp/src/SimulationInternal.hpp:259-267). FVG_EINVAL, FVG_ELOGIC and FVG_EDIVERGED carry those three.
+ *
+ * A context is one device plus one stream. It is not thread-safe: calls on one context must not overlap
+ * (the reference's Simulation is not re-entrant either, cpp/include/fv/Physics.hpp:118-119).
+ */
+#ifndef FVG_H
+#define FVG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fvg_status {
+    FVG_OK = 0,
+    FVG_EINVAL = 1,     /* bad name, value or size (std::invalid_argument in the reference) */
+    FVG_ELOGIC = 2,     /* contract violation, e.g. a halo too shallow for the stencil (std::logic_error) */
+    FVG_EDIVERGED = 3,  /* non-finite wave speed or time step, or t + dt == t (check_progress, std::runtime_error) */
+    FVG_EHIP = 4,       /* a HIP runtime call failed */
+    FVG_ENOMEM = 5,     /* device memory could not be allocated */
+    FVG_EUNSUPPORTED = 6 /* a configuration the device path does not implement (the message names it) */
+} fvg_status;
+
+/* Boundary-condition types, in the order of the reference's enum BCType (cpp/include/fv/Boundaries.hpp:15). */
+enum fvg_bc {
+    FVG_BC_WALL = 0,
+    FVG_BC_NO_SLIP = 1,
+    FVG_BC_MOVING_WALL = 2,
+    FVG_BC_VEL_INLET = 3,
+    FVG_BC_MASSFLOW_INLET = 4,
+    FVG_BC_OUTLET = 5,
+    FVG_BC_TRANSMISSIVE = 6,
+    FVG_BC_PERIODIC = 7,
+    FVG_BC_NONREFLECTING = 8
+};
+/* Sides, in the order of the reference's enum Face (cpp/include/fv/Core.hpp:44). */
+enum fvg_side { FVG_LEFT = 0, FVG_RIGHT = 1, FVG_BOTTOM = 2, FVG_TOP = 3, FVG_FRONT = 4, FVG_BACK = 5 };
+enum fvg_riemann { FVG_HLLC = 0, FVG_LOWMACH_HLLC = 1 };                 /* Schemes.hpp:16 */
+enum fvg_recon { FVG_RECON_NONE = 0, FVG_RECON_MUSCL = 1, FVG_RECON_WENO3 = 2, FVG_RECON_WENO5 = 3 };
+enum fvg_limiter { FVG_LIM_MINMOD = 0, FVG_LIM_VAN_LEER = 1, FVG_LIM_MC = 2, FVG_LIM_KOREN = 3 }; /* Schemes.hpp:82 */
+enum fvg_time_scheme { FVG_TS_AUTO = -1, FVG_TS_EULER = 0, FVG_TS_SSPRK2 = 1, FVG_TS_SSPRK3 = 2, FVG_TS_HANCOCK = 3 };
+enum fvg_eos { FVG_EOS_IDEAL = 0, FVG_EOS_STIFFENED = 1 };
+/* Arithmetic of the reconstruction and the fluxes. FP64 is the reference's. FP32 evaluates reconstruction,
+ * Riemann solver and viscous fluxes in float, while the state, the stage combinations, the time step and the
+ * conservation sums stay FP64. */
+enum fvg_flux_precision { FVG_FLUX_FP64 = 0, FVG_FLUX_FP32 = 1 };
+
+/* The keys of the path (cpp/src/Params.cpp:98-223 and SimParams::from), typed. Defaults: fvg_config_defaults. */
+typedef struct fvg_config {
+    int nx, ny, nz;              /* cells; nz = 1 is a 2-D mesh */
+    double Lx, Ly, Lz;           /* extent */
+    int eos_type;                /* enum fvg_eos */
+    double gamma, p_inf;
+    int bc[6];                   /* enum fvg_bc per side, enum fvg_side order (bc_xmin, bc_xmax, bc_ymin, bc_ymax, bc_zmin, bc_zmax) */
+    double u_in, v_in, rho_in, p_out; /* BC parameters (moving wall, inlets, outlet) */
+    int riemann;                 /* enum fvg_riemann */
+    double shock_blend;          /* default 0.2 */
+    int reconstruction;          /* enum fvg_recon */
+    int limiter;                 /* enum fvg_limiter */
+    double weno_epsilon;         /* default 1e-6 */
+    int boundary_order;          /* 1 or 2 (default 2) */
+    int time_scheme;             /* enum fvg_time_scheme; FVG_TS_AUTO is resolved by fvg_config_finalize */
+    double cfl;                  /* default 0.5 */
+    int use_viscous;
+    double mu, pr;               /* defaults 1e-3, 0.72 */
+    double t_end, dt_max;        /* run keys */
+    int max_steps;
+    int flux_precision;          /* enum fvg_flux_precision (no reference key; default FP64) */
+} fvg_config;
+
+/* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,
+ * nz, Lx, Ly, Lz are set to 0 / 0.0 / 1 / 1.0 and must be given. */
+void fvg_config_defaults(fvg_config* cfg);
+/* Set one key from its text value, with the reference's key names and case-insensitive labels and aliases
+ * ("HLLC (Low-Mach Thornber)", "lowmach", "Van Leer", "SSP Runge-Kutta 3", "periodic", ...). Replaces
+ * SimParams::from + parse_riemann / parse_limiter / parse_reconstruction / parse_time_scheme / parse_bc
+ * (Params.cpp, Schemes.cpp:77-102, TimeScheme.cpp:78-85, Boundaries.cpp:15-27).
+ * Returns FVG_EINVAL with a message in err (may be NULL) for an unknown key or label. */
+int fvg_config_set(fvg_config* cfg, const char* key, const char* value, char* err, size_t err_len);
+/* Resolve time_scheme = auto (Params.cpp: MUSCL-Hancock for inviscid MUSCL, SSP-RK2 for viscous MUSCL,
+ * Explicit Euler otherwise) and check sizes. */
+int fvg_config_finalize(fvg_config* cfg, char* err, size_t err_len);
+/* The CFL length L of dt = cfl L / (S + 1e-10): min(dx, dy) in 2-D, 2 / (1/dx + 1/dy + 1/dz) in 3-D
+ * (Simulation.cpp:223-227, ViscousBound.hpp:28-29). */
+double fvg_dt_length(const fvg_config* cfg);
+
+typedef struct fvg_ctx fvg_ctx;
+
+/* A context on HIP device `device` for the mesh and numerics of cfg: allocates the device state and work
+ * buffers. Replaces Simulation::setup for the uniform path (Simulation.cpp:188). */
+int fvg_create(const fvg_config* cfg, int device, fvg_ctx** out);
+int fvg_destroy(fvg_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed fvg_create when ctx is NULL). */
+const char* fvg_last_error(const fvg_ctx* ctx);
+
+/* Host AoS state in / out (ncomp 4 or 5, ncells = nx*ny*nz). Synchronous. */
+int fvg_upload(fvg_ctx* ctx, const double* U, size_t ncells, int ncomp);
+int fvg_download(fvg_ctx* ctx, double* U, size_t ncells, int ncomp);
+
+/* R = residual(U) of the resident state and the largest wave speed it saw: Simulation::residual
+ * (Simulation.cpp:1351) as the ResidualFn of TimeScheme.hpp:14. predictor_half_dt > 0 runs the
+ * MUSCL-Hancock predictor inside it (TimeScheme.hpp:92). R (may be NULL) receives the host AoS residual
+ * with ncomp components. */
+int fvg_residual(fvg_ctx* ctx, double predictor_half_dt, double* R, int ncomp, double* s_max);
+
+/* One step of the configured scheme on the resident state: TimeScheme::step(U, residual, dt_length, t,
+ * t_max, dt_max) -> {dt, s_max} (TimeScheme.hpp:17-20,48), followed by the reference's divergence check
+ * (check_progress). Returns FVG_EDIVERGED (state left as the step made it) when the check fails. */
+int fvg_step(fvg_ctx* ctx, double t, double t_max, double dt_max, double* dt, double* s_max);
+
+/* Up to max_steps steps from time *t toward t_end without a host round trip per step (time, dt and the
+ * divergence flag stay on the device; the run loop condition is `t < t_end`, SimulationRun.cpp:167).
+ * max_steps <= 0: no step limit. On return *t is the new time and *steps the number of steps taken; s_last
+ * (may be NULL) gets the last step's wave speed. */
+int fvg_advance(fvg_ctx* ctx, int max_steps, double* t, double t_end, double dt_max, int* steps, double* s_last);
+
+/* The MUSCL-Hancock lagged wave speed (TimeScheme.hpp:96-97; checkpoint key "S"). */
+int fvg_get_lagged_speed(const fvg_ctx* ctx, double* s);
+int fvg_set_lagged_speed(fvg_ctx* ctx, double s);
+
+/* Volume averages of mass, kinetic energy and total energy (Simulation::volume_averages,
+ * SimulationRun.cpp:45-64). The device sum is a fixed-order FP64 tree, so it matches the reference's
+ * sequential sum to round-off, not bit for bit. */
+int fvg_conserved_totals(fvg_ctx* ctx, double out[3]);
+
+/* Timing of the kernels of the last fvg_advance / fvg_step calls, measured with HIP events on the
+ * context's stream: name[k] (static strings), total milliseconds and launch count per kernel family.
+ * Returns the number of families written (at most cap). */
+int fvg_kernel_times(const fvg_ctx* ctx, const char** names, double* ms, long long* launches, int cap);
+/* Switch the per-kernel event timing on (1) or off (0, the default: no events are recorded). */
+int fvg_set_timing(fvg_ctx* ctx, int on);
+
+/* Known-answer entry: the Riemann flux of n faces with host AoS conservative states (5 doubles per face),
+ * axis per face, through the device kernel of the solver configured in cfg (riemann, shock_blend, EOS).
+ * F receives 5 doubles per face and smax one. Replaces RiemannSolver::face_flux_batch (Schemes.hpp:43). */
+int fvg_riemann_batch(const fvg_config* cfg, int device, const double* UL, const double* UR, const int* axis, size_t n,
+                      double* F, double* smax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVG_H */
