@@ -1,0 +1,644 @@
+// Host side of the C ABI (include/fvg.h): a context owns one device, one stream and the device buffers of one
+// mesh; residual / step / advance enqueue the kernels of fvg_kernels.hip and keep time, dt and the divergence flag
+// on the device. Every HIP call is checked; failures come back as status codes with a message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/fvg.h"
+#include "fvg_kernels.hpp"
+
+namespace fvg {
+template <int NV> __global__ void k_cells(KP);
+template <int NV> __global__ void k_hancock(KP);
+template <int NV, int D, int MODE> __global__ void k_faces(KP);
+template <int NV, int UPD> __global__ void k_update(KP, const double*, const double*, double*);
+template <int NV> __global__ void k_totals_partial(const double*, long long, double*);
+__global__ void k_totals_final(const double*, int, long long, double*);
+__global__ void k_control(KP, int, int);
+__global__ void k_riemann_batch(RiemannParams, const double*, const double*, const int*, long long, double*, double*);
+}   // namespace fvg
+
+using namespace fvg;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTotalsBlocks = 1024;
+thread_local std::string g_create_error;
+
+enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_N };
+const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control"};
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+}   // namespace
+
+struct fvg_ctx {
+    fvg_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int NV = 5, ND = 3;
+    long long nc = 0, nfa[3] = {0, 0, 0};
+    double *U = nullptr, *Ut = nullptr, *Rb = nullptr, *W = nullptr, *G = nullptr, *HS = nullptr;
+    double *Fc[3] = {nullptr, nullptr, nullptr}, *Fv[3] = {nullptr, nullptr, nullptr};
+    double *tot_part = nullptr, *tot_out = nullptr;
+    DevState* st = nullptr;
+    DevState* st_host = nullptr;   // pinned mirror for the read-back
+    KP base{};
+    bool lagged_positive = false;
+    std::string err;
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec {
+        int fam;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    double fam_ms[FAM_N] = {};
+    long long fam_n[FAM_N] = {};
+};
+
+namespace {
+
+int set_err(fvg_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    else g_create_error = m;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                         \
+    do {                                                                                                          \
+        hipError_t e_ = (call);                                                                                   \
+        if (e_ != hipSuccess) return set_err(ctx, FVG_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+hipEvent_t take_event(fvg_ctx* c) {
+    hipEvent_t e;
+    if (!c->ev_pool.empty()) {
+        e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch with optional event timing of the kernel family.
+template <typename F>
+int launch(fvg_ctx* c, int fam, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->timing) {
+        a = take_event(c);
+        b = take_event(c);
+        if (a) hipEventRecord(a, c->stream);
+    }
+    f();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, FVG_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (c->timing && a && b) {
+        hipEventRecord(b, c->stream);
+        c->recs.push_back({fam, a, b});
+    }
+    return FVG_OK;
+}
+
+int harvest_timing(fvg_ctx* c) {
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->fam_ms[r.fam] += ms;
+            c->fam_n[r.fam] += 1;
+        }
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->recs.clear();
+    return FVG_OK;
+}
+
+KP make_kp(fvg_ctx* c, const double* Uin) {
+    KP p = c->base;
+    p.U = Uin;
+    return p;
+}
+
+template <int NV>
+int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode) {
+    const KP p = make_kp(c, Uin);
+    int rc;
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL(k_cells<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+        return rc;
+    if (mode == MODE_HANCOCK)
+        if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+            return rc;
+#define FACES(D, M)                                                                                                                       \
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, M>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+#define AXIS(D)                                                                             \
+    (mode == MODE_PLAIN ? FACES(D, MODE_PLAIN) : (mode == MODE_HANCOCK ? FACES(D, MODE_HANCOCK) : FACES(D, MODE_FIRST_ORDER)))
+    if ((rc = AXIS(0))) return rc;
+    if ((rc = AXIS(1))) return rc;
+    if constexpr (NV == 5)
+        if ((rc = AXIS(2))) return rc;
+#undef AXIS
+#undef FACES
+    return FVG_OK;
+}
+
+int enqueue_residual(fvg_ctx* c, const double* Uin, int mode) {
+    return c->NV == 5 ? enqueue_residual_nv<5>(c, Uin, mode) : enqueue_residual_nv<4>(c, Uin, mode);
+}
+
+int enqueue_control(fvg_ctx* c, int op, int arg) {
+    const KP p = c->base;
+    return launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, c->stream, p, op, arg); });
+}
+
+template <int NV>
+int enqueue_update_nv(fvg_ctx* c, int upd, const double* U0, const double* Ut, double* out) {
+    const KP p = c->base;
+    const dim3 g(grid_for(c->nc)), b(kBlock);
+    return launch(c, FAM_UPDATE, [&] {
+        switch (upd) {
+        case UPD_RESIDUAL: hipLaunchKernelGGL((k_update<NV, UPD_RESIDUAL>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_AXPY: hipLaunchKernelGGL((k_update<NV, UPD_AXPY>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_RK2_2: hipLaunchKernelGGL((k_update<NV, UPD_RK2_2>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_RK3_2: hipLaunchKernelGGL((k_update<NV, UPD_RK3_2>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        default: hipLaunchKernelGGL((k_update<NV, UPD_RK3_3>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        }
+    });
+}
+int enqueue_update(fvg_ctx* c, int upd, const double* U0, const double* Ut, double* out) {
+    return c->NV == 5 ? enqueue_update_nv<5>(c, upd, U0, Ut, out) : enqueue_update_nv<4>(c, upd, U0, Ut, out);
+}
+
+int plain_mode(const fvg_ctx* c) { return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN; }
+
+// One step of the configured scheme, enqueued (TimeScheme.cpp:110-319).
+int enqueue_step(fvg_ctx* c) {
+    int rc;
+    const int ts = c->cfg.time_scheme;
+    if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
+    if (ts == FVG_TS_HANCOCK) {
+        const int mode = c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_HANCOCK;
+        if (!c->lagged_positive) {   // no wave speed yet: one plain residual gives it (TimeScheme.cpp:298-302)
+            if ((rc = enqueue_residual(c, c->U, plain_mode(c)))) return rc;
+            if ((rc = enqueue_control(c, CTL_SEED, ts))) return rc;
+        }
+        if ((rc = enqueue_control(c, CTL_HANCOCK_DT, ts))) return rc;
+        if ((rc = enqueue_residual(c, c->U, mode))) return rc;
+        if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
+        if ((rc = enqueue_control(c, CTL_END_HANCOCK, ts))) return rc;
+        c->lagged_positive = true;
+        return FVG_OK;
+    }
+    const int mode = plain_mode(c);
+    if ((rc = enqueue_residual(c, c->U, mode))) return rc;
+    if ((rc = enqueue_control(c, CTL_STAGE1_DT, ts))) return rc;
+    if (ts == FVG_TS_EULER) {
+        if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
+        return enqueue_control(c, CTL_END_RK, ts);
+    }
+    if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->Ut))) return rc;
+    if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
+    if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
+    if (ts == FVG_TS_SSPRK2) {
+        if ((rc = enqueue_update(c, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
+        return enqueue_control(c, CTL_END_RK, ts);
+    }
+    if ((rc = enqueue_update(c, UPD_RK3_2, c->U, c->Ut, c->Ut))) return rc;
+    if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
+    if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
+    if ((rc = enqueue_update(c, UPD_RK3_3, c->U, c->Ut, c->U))) return rc;
+    return enqueue_control(c, CTL_END_RK, ts);
+}
+
+int write_state(fvg_ctx* c, const DevState& s) {
+    *c->st_host = s;
+    HIPCHK(c, hipMemcpyAsync(c->st, c->st_host, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    return FVG_OK;
+}
+int read_state(fvg_ctx* c, DevState& s) {
+    HIPCHK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s = *c->st_host;
+    harvest_timing(c);
+    return FVG_OK;
+}
+
+DevState fresh_state(const fvg_ctx* c) {
+    DevState s{};
+    s.t = 0.0;
+    s.t_end = std::numeric_limits<double>::infinity();
+    s.t_max = std::numeric_limits<double>::infinity();
+    s.lagged = c->st_host ? c->st_host->lagged : 0.0;
+    return s;
+}
+
+void free_ctx(fvg_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    double* bufs[] = {c->U, c->Ut, c->Rb, c->W, c->G, c->HS, c->Fc[0], c->Fc[1], c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out};
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    if (c->st) hipFree(c->st);
+    if (c->st_host) hipHostFree(c->st_host);
+    for (auto& r : c->recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int checked_alloc(fvg_ctx* c, double** p, size_t n) {
+    if (n == 0) return FVG_OK;
+    hipError_t e = hipMalloc(p, n * sizeof(double));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, FVG_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes: " + hipGetErrorString(e));
+    }
+    return FVG_OK;
+}
+
+}   // namespace
+
+extern "C" {
+
+const char* fvg_last_error(const fvg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
+    if (!cfg_in || !out) return set_err(nullptr, FVG_EINVAL, "null argument");
+    *out = nullptr;
+    fvg_config cfg = *cfg_in;
+    char msg[256] = {0};
+    if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return set_err(nullptr, FVG_EINVAL, msg);
+    for (int s = 0; s < 6; ++s) {
+        const int b = cfg.bc[s];
+        if ((s < 4 || cfg.nz > 1) && (b == FVG_BC_NONREFLECTING || b == FVG_BC_MASSFLOW_INLET))
+            return set_err(nullptr, FVG_EUNSUPPORTED, "boundary type of side " + std::to_string(s) +
+                                                          " (non-reflecting / mass-flow inlet) is not on the device path");
+    }
+    if (cfg.time_scheme == FVG_TS_HANCOCK && (cfg.reconstruction == FVG_RECON_WENO3 || cfg.reconstruction == FVG_RECON_WENO5))
+        return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
+    if (cfg.flux_precision != FVG_FLUX_FP64)
+        return set_err(nullptr, FVG_EUNSUPPORTED, "flux_precision = fp32 is not on the device path yet");
+    const double faces = (double)(cfg.nx + 1) * cfg.ny * cfg.nz;
+    if (faces * 64 > 9.0e18) return set_err(nullptr, FVG_EINVAL, "mesh too large");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(nullptr, FVG_EHIP, "no HIP device");
+    if (device < 0 || device >= ndev) return set_err(nullptr, FVG_EINVAL, "device index out of range");
+    fvg_ctx* c = new fvg_ctx();
+    c->cfg = cfg;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        free_ctx(c);
+        return set_err(nullptr, FVG_EHIP, "hipSetDevice failed");
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return set_err(nullptr, FVG_EHIP, "hipStreamCreate failed");
+    }
+    const bool three = cfg.nz > 1;
+    c->NV = three ? 5 : 4;
+    c->ND = three ? 3 : 2;
+    c->nc = (long long)cfg.nx * cfg.ny * cfg.nz;
+    c->nfa[0] = (long long)(cfg.nx + 1) * cfg.ny * cfg.nz;
+    c->nfa[1] = (long long)cfg.nx * (cfg.ny + 1) * cfg.nz;
+    c->nfa[2] = three ? (long long)cfg.nx * cfg.ny * (cfg.nz + 1) : 0;
+    const int NV = c->NV, ND = c->ND;
+    const bool visc = cfg.use_viscous != 0, hancock = cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE;
+    int rc = FVG_OK;
+    rc = rc ? rc : checked_alloc(c, &c->U, (size_t)NV * c->nc);
+    rc = rc ? rc : checked_alloc(c, &c->Ut, (size_t)NV * c->nc);
+    rc = rc ? rc : checked_alloc(c, &c->Rb, (size_t)NV * c->nc);
+    rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * c->nc);
+    if (visc) rc = rc ? rc : checked_alloc(c, &c->G, (size_t)ND * ND * c->nc);
+    if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
+    for (int d = 0; d < ND; ++d) {
+        rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d]);
+        if (visc) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d]);
+    }
+    rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
+    rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
+    if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
+    if (!rc && hipHostMalloc(&c->st_host, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipHostMalloc of the state mirror");
+    if (rc) {
+        const std::string m = c->err;
+        free_ctx(c);
+        return set_err(nullptr, rc, m);
+    }
+    std::memset(c->st_host, 0, sizeof(DevState));
+
+    // kernel parameters that do not change during the run
+    KP& p = c->base;
+    p.nx = cfg.nx;
+    p.ny = cfg.ny;
+    p.nz = cfg.nz;
+    p.is3d = three;
+    p.nc = c->nc;
+    p.dx = cfg.Lx / cfg.nx;
+    p.dy = cfg.Ly / cfg.ny;
+    p.dz = cfg.Lz / cfg.nz;
+    if (three) {   // Mesh::face_area / cell_volume (Core.hpp:94-99)
+        p.Ax = p.dy * p.dz;
+        p.Ay = p.dx * p.dz;
+        p.Az = p.dx * p.dy;
+        p.inv_vol = 1.0 / (p.dx * p.dy * p.dz);
+    } else {
+        p.Ax = p.dy;
+        p.Ay = p.dx;
+        p.Az = 0.0;
+        p.inv_vol = 1.0 / (p.dx * p.dy);
+    }
+    p.rs.eos.stiff = cfg.eos_type == FVG_EOS_STIFFENED;
+    p.rs.eos.gamma = cfg.gamma;
+    p.rs.eos.p_inf = p.rs.eos.stiff ? cfg.p_inf : 0.0;
+    p.rs.lowmach = cfg.riemann == FVG_LOWMACH_HLLC;
+    p.rs.blend = cfg.shock_blend;
+    for (int s = 0; s < 6; ++s) p.bp.bc[s] = cfg.bc[s];
+    p.bp.u_in = cfg.u_in;
+    p.bp.v_in = cfg.v_in;
+    p.bp.p_out = cfg.p_out;
+    p.recon = cfg.reconstruction;
+    p.lim = cfg.limiter;
+    p.Dh = cfg.reconstruction == FVG_RECON_WENO5 ? 3 : 2;
+    p.bo2 = cfg.boundary_order == 2;
+    p.weno_eps = cfg.weno_epsilon;
+    // halos of the reconstruction (Simulation.cpp:863-905, 1109-1170)
+    const int n3[3] = {cfg.nx, cfg.ny, cfg.nz};
+    for (int a = 0; a < 3; ++a) {
+        p.wrap[a] = 0;
+        p.halo[2 * a] = p.halo[2 * a + 1] = 0;
+        if (a == 2 && !three) continue;
+        if (cfg.reconstruction == FVG_RECON_NONE) continue;
+        const bool per = cfg.bc[2 * a] == FVG_BC_PERIODIC && cfg.bc[2 * a + 1] == FVG_BC_PERIODIC;
+        if (per && n3[a] >= p.Dh) {
+            p.wrap[a] = 1;
+            p.halo[2 * a] = p.halo[2 * a + 1] = 1;
+            continue;
+        }
+        if (cfg.boundary_order == 2)
+            for (int hi = 0; hi < 2; ++hi) {
+                const int bt = cfg.bc[2 * a + hi];
+                if (bt != FVG_BC_PERIODIC && bt != FVG_BC_NONREFLECTING && n3[a] >= p.Dh) p.halo[2 * a + hi] = 2;
+            }
+    }
+    p.cfl = cfg.cfl;
+    p.Ldt = fvg_dt_length(&cfg);
+    p.viscous = visc;
+    p.mu = cfg.mu;
+    p.k_cond = cfg.mu * cfg.gamma / (cfg.pr * (cfg.gamma - 1.0));   // EOS::TransportLaw (Core.hpp:196)
+    p.inv_h2 = three ? 1.0 / (p.dx * p.dx) + 1.0 / (p.dy * p.dy) + 1.0 / (p.dz * p.dz) : 1.0 / (p.dx * p.dx) + 1.0 / (p.dy * p.dy);
+    for (int a = 0; a < 3; ++a) p.vper[a] = cfg.bc[2 * a] == FVG_BC_PERIODIC && cfg.bc[2 * a + 1] == FVG_BC_PERIODIC;
+    p.U = c->U;
+    p.W = c->W;
+    p.G = c->G;
+    p.HS = c->HS;
+    for (int d = 0; d < 3; ++d) {
+        p.Fc[d] = c->Fc[d];
+        p.Fv[d] = c->Fv[d];
+        p.nfa[d] = c->nfa[d];
+    }
+    p.st = c->st;
+    DevState s = fresh_state(c);
+    if ((rc = write_state(c, s)) || (rc = (hipStreamSynchronize(c->stream) == hipSuccess ? 0 : FVG_EHIP))) {
+        const std::string m = c->err;
+        free_ctx(c);
+        return set_err(nullptr, rc, m);
+    }
+    *out = c;
+    return FVG_OK;
+}
+
+int fvg_destroy(fvg_ctx* c) {
+    free_ctx(c);
+    return FVG_OK;
+}
+
+int fvg_upload(fvg_ctx* c, const double* U, size_t ncells, int ncomp) {
+    if (!c || !U) return set_err(c, FVG_EINVAL, "null argument");
+    if ((long long)ncells != c->nc) return set_err(c, FVG_EINVAL, "ncells does not match the mesh");
+    if (ncomp != 5 && !(ncomp == 4 && c->NV == 4)) return set_err(c, FVG_EINVAL, "ncomp must be 5 (or 4 on a 2-D mesh)");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<double> soa((size_t)c->NV * c->nc);
+    for (long long x = 0; x < c->nc; ++x)
+        for (int q = 0; q < c->NV; ++q) soa[(size_t)q * c->nc + x] = U[(size_t)x * ncomp + q];
+    HIPCHK(c, hipMemcpyAsync(c->U, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+static int download_planes(fvg_ctx* c, const double* src, double* U, int ncomp) {
+    std::vector<double> soa((size_t)c->NV * c->nc);
+    HIPCHK(c, hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long long x = 0; x < c->nc; ++x) {
+        for (int q = 0; q < c->NV; ++q) U[(size_t)x * ncomp + q] = soa[(size_t)q * c->nc + x];
+        if (c->NV == 4 && ncomp == 5) U[(size_t)x * 5 + 4] = 0.0;   // rho*w of a 2-D field
+    }
+    return FVG_OK;
+}
+
+int fvg_download(fvg_ctx* c, double* U, size_t ncells, int ncomp) {
+    if (!c || !U) return set_err(c, FVG_EINVAL, "null argument");
+    if ((long long)ncells != c->nc) return set_err(c, FVG_EINVAL, "ncells does not match the mesh");
+    if (ncomp != 5 && !(ncomp == 4 && c->NV == 4)) return set_err(c, FVG_EINVAL, "ncomp must be 5 (or 4 on a 2-D mesh)");
+    HIPCHK(c, hipSetDevice(c->device));
+    return download_planes(c, c->U, U, ncomp);
+}
+
+int fvg_residual(fvg_ctx* c, double half_dt, double* R, int ncomp, double* s_max) {
+    if (!c) return set_err(c, FVG_EINVAL, "null context");
+    if (R && ncomp != 5 && !(ncomp == 4 && c->NV == 4)) return set_err(c, FVG_EINVAL, "ncomp must be 5 (or 4 on a 2-D mesh)");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    DevState s = fresh_state(c);
+    const bool hancock = half_dt > 0.0 && c->cfg.reconstruction != FVG_RECON_NONE;
+    if (hancock && !c->HS) {
+        if ((rc = checked_alloc(c, &c->HS, (size_t)2 * c->ND * c->NV * c->nc))) return rc;
+        c->base.HS = c->HS;
+    }
+    s.half_dt = hancock ? half_dt : 0.0;
+    if ((rc = write_state(c, s))) return rc;
+    if ((rc = enqueue_control(c, CTL_BEGIN_STEP, 0))) return rc;
+    if ((rc = enqueue_residual(c, c->U, hancock ? MODE_HANCOCK : plain_mode(c)))) return rc;
+    if ((rc = enqueue_control(c, CTL_RESIDUAL_S, 0))) return rc;
+    if ((rc = enqueue_update(c, UPD_RESIDUAL, nullptr, nullptr, c->Rb))) return rc;
+    DevState o;
+    if ((rc = read_state(c, o))) return rc;
+    if (s_max) *s_max = o.s_step;
+    if (R) return download_planes(c, c->Rb, R, ncomp);
+    return FVG_OK;
+}
+
+int fvg_step(fvg_ctx* c, double t, double t_max, double dt_max, double* dt, double* s_max) {
+    if (!c) return set_err(c, FVG_EINVAL, "null context");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    DevState s = fresh_state(c);
+    s.t = t;
+    s.t_max = t_max;
+    s.dt_max = dt_max;
+    s.max_steps = 1;
+    if ((rc = write_state(c, s))) return rc;
+    if ((rc = enqueue_step(c))) return rc;
+    DevState o;
+    if ((rc = read_state(c, o))) return rc;
+    if (dt) *dt = o.dt;
+    if (s_max) *s_max = o.s_step;
+    if (o.diverged) {
+        char m[256];
+        std::snprintf(m, sizeof m, "the solution diverged (t = %.6g, dt = %.3g, largest wave speed = %.3g)", t, o.dt, o.s_step);
+        return set_err(c, FVG_EDIVERGED, m);
+    }
+    return FVG_OK;
+}
+
+int fvg_advance(fvg_ctx* c, int max_steps, double* t, double t_end, double dt_max, int* steps, double* s_last) {
+    if (!c || !t) return set_err(c, FVG_EINVAL, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    DevState s = fresh_state(c);
+    s.t = *t;
+    s.t_end = t_end;
+    s.t_max = t_end;
+    s.dt_max = dt_max;
+    s.max_steps = max_steps > 0 ? max_steps : 0;
+    if ((rc = write_state(c, s))) return rc;
+    DevState o{};
+    long long enqueued = 0;
+    const int chunk = 16;
+    for (;;) {
+        const long long left = max_steps > 0 ? max_steps - enqueued : chunk;
+        const int n = (int)std::min<long long>(chunk, left);
+        for (int k = 0; k < n; ++k)
+            if ((rc = enqueue_step(c))) return rc;
+        enqueued += n;
+        if ((rc = read_state(c, o))) return rc;
+        if (o.done || o.diverged || (max_steps > 0 && enqueued >= max_steps)) break;
+    }
+    *t = o.t;
+    if (steps) *steps = o.steps;
+    if (s_last) *s_last = o.s_step;
+    if (o.diverged) {
+        char m[256];
+        std::snprintf(m, sizeof m, "the solution diverged at step %d (t = %.6g, dt = %.3g, largest wave speed = %.3g)", o.steps, o.t,
+                      o.dt, o.s_step);
+        return set_err(c, FVG_EDIVERGED, m);
+    }
+    return FVG_OK;
+}
+
+int fvg_get_lagged_speed(const fvg_ctx* c, double* s) {
+    if (!c || !s) return FVG_EINVAL;
+    *s = c->st_host->lagged;
+    return FVG_OK;
+}
+
+int fvg_set_lagged_speed(fvg_ctx* c, double sp) {
+    if (!c) return FVG_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    c->st_host->lagged = sp;
+    DevState s = fresh_state(c);
+    s.lagged = sp;
+    int rc = write_state(c, s);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lagged_positive = sp > 0.0;
+    return FVG_OK;
+}
+
+int fvg_conserved_totals(fvg_ctx* c, double out[3]) {
+    if (!c || !out) return set_err(c, FVG_EINVAL, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nb = kTotalsBlocks;
+    if (c->NV == 5) hipLaunchKernelGGL(k_totals_partial<5>, dim3(nb), dim3(256), 0, c->stream, c->U, c->nc, c->tot_part);
+    else hipLaunchKernelGGL(k_totals_partial<4>, dim3(nb), dim3(256), 0, c->stream, c->U, c->nc, c->tot_part);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, c->stream, c->tot_part, nb, c->nc, c->tot_out);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, c->tot_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_set_timing(fvg_ctx* c, int on) {
+    if (!c) return FVG_EINVAL;
+    c->timing = on != 0;
+    for (int f = 0; f < FAM_N; ++f) {
+        c->fam_ms[f] = 0.0;
+        c->fam_n[f] = 0;
+    }
+    return FVG_OK;
+}
+
+int fvg_kernel_times(const fvg_ctx* c, const char** names, double* ms, long long* launches, int cap) {
+    if (!c) return 0;
+    int k = 0;
+    for (int f = 0; f < FAM_N && k < cap; ++f) {
+        if (names) names[k] = kFamilyNames[f];
+        if (ms) ms[k] = c->fam_ms[f];
+        if (launches) launches[k] = c->fam_n[f];
+        ++k;
+    }
+    return k;
+}
+
+int fvg_riemann_batch(const fvg_config* cfg, int device, const double* UL, const double* UR, const int* axis, size_t n, double* F,
+                      double* smax) {
+    if (!cfg || !UL || !UR || !axis || !F || !smax) return set_err(nullptr, FVG_EINVAL, "null argument");
+    if (n == 0) return FVG_OK;
+    for (size_t k = 0; k < n; ++k)
+        if (axis[k] < 0 || axis[k] > 2) return set_err(nullptr, FVG_EINVAL, "axis must be 0, 1 or 2");
+    if (hipSetDevice(device) != hipSuccess) return set_err(nullptr, FVG_EHIP, "hipSetDevice failed");
+    RiemannParams rs;
+    rs.eos.stiff = cfg->eos_type == FVG_EOS_STIFFENED;
+    rs.eos.gamma = cfg->gamma;
+    rs.eos.p_inf = rs.eos.stiff ? cfg->p_inf : 0.0;
+    rs.lowmach = cfg->riemann == FVG_LOWMACH_HLLC;
+    rs.blend = cfg->shock_blend;
+    double *dUL = nullptr, *dUR = nullptr, *dF = nullptr, *dS = nullptr;
+    int* dA = nullptr;
+    int rc = FVG_OK;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return set_err(nullptr, FVG_EHIP, "hipStreamCreate failed");
+    if (hipMalloc(&dUL, 5 * n * sizeof(double)) != hipSuccess || hipMalloc(&dUR, 5 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dF, 5 * n * sizeof(double)) != hipSuccess || hipMalloc(&dS, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dA, n * sizeof(int)) != hipSuccess) {
+        rc = set_err(nullptr, FVG_ENOMEM, "hipMalloc failed");
+    }
+    if (!rc) {
+        hipMemcpyAsync(dUL, UL, 5 * n * sizeof(double), hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(dUR, UR, 5 * n * sizeof(double), hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(dA, axis, n * sizeof(int), hipMemcpyHostToDevice, st);
+        hipLaunchKernelGGL(k_riemann_batch, dim3(grid_for((long long)n)), dim3(kBlock), 0, st, rs, dUL, dUR, dA, (long long)n, dF, dS);
+        hipMemcpyAsync(F, dF, 5 * n * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipMemcpyAsync(smax, dS, n * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) rc = set_err(nullptr, FVG_EHIP, std::string("riemann batch: ") + hipGetErrorString(e));
+    }
+    if (dUL) hipFree(dUL);
+    if (dUR) hipFree(dUR);
+    if (dF) hipFree(dF);
+    if (dS) hipFree(dS);
+    if (dA) hipFree(dA);
+    hipStreamDestroy(st);
+    return rc;
+}
+
+}   // extern "C"

@@ -1,0 +1,211 @@
+// Run keys of the uniform-grid path: defaults, key = value parsing with the reference's labels and aliases,
+// and the `auto` time-scheme rule. Host code only.
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/fvg.h"
+
+namespace {
+
+std::string lower(std::string s) {
+    for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    return s;
+}
+std::string trim(const std::string& s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace(static_cast<unsigned char>(s[a]))) ++a;
+    while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) --b;
+    return s.substr(a, b - a);
+}
+int fail(char* err, size_t n, const std::string& msg) {
+    if (err && n) std::snprintf(err, n, "%s", msg.c_str());
+    return FVG_EINVAL;
+}
+bool parse_num(const std::string& v, double& out) {
+    const std::string t = trim(v);
+    if (t.empty()) return false;
+    char* end = nullptr;
+    out = std::strtod(t.c_str(), &end);
+    return end && *end == '\0';
+}
+// Params::integer: "100" or "100.0"
+bool parse_int(const std::string& v, int& out) {
+    double d;
+    if (!parse_num(v, d) || d != std::floor(d)) return false;
+    out = static_cast<int>(d);
+    return true;
+}
+// Params::flag: true/false, 1/0, yes/no, on/off
+bool parse_flag(const std::string& v, int& out) {
+    const std::string s = lower(trim(v));
+    if (s == "true" || s == "1" || s == "yes" || s == "on") return (out = 1), true;
+    if (s == "false" || s == "0" || s == "no" || s == "off") return (out = 0), true;
+    return false;
+}
+
+}   // namespace
+
+extern "C" {
+
+void fvg_config_defaults(fvg_config* c) {
+    std::memset(c, 0, sizeof *c);
+    c->nx = 0;
+    c->ny = 0;
+    c->nz = 1;
+    c->Lx = c->Ly = c->Lz = 1.0;
+    c->eos_type = FVG_EOS_IDEAL;
+    c->gamma = 1.4;
+    c->p_inf = 0.0;
+    // BoundaryManager's sides before any case sets them (Boundaries.hpp:128-129)
+    const int bc0[6] = {FVG_BC_TRANSMISSIVE, FVG_BC_TRANSMISSIVE, FVG_BC_WALL, FVG_BC_WALL, FVG_BC_PERIODIC, FVG_BC_PERIODIC};
+    for (int s = 0; s < 6; ++s) c->bc[s] = bc0[s];
+    c->u_in = 0.0;
+    c->v_in = 0.0;
+    c->rho_in = 1.0;
+    c->p_out = 1.0;
+    c->riemann = FVG_HLLC;
+    c->shock_blend = 0.2;
+    c->reconstruction = FVG_RECON_NONE;
+    c->limiter = FVG_LIM_VAN_LEER;
+    c->weno_epsilon = 1e-6;
+    c->boundary_order = 2;
+    c->time_scheme = FVG_TS_AUTO;
+    c->cfl = 0.5;
+    c->use_viscous = 0;
+    c->mu = 1e-3;
+    c->pr = 0.72;
+    c->t_end = 0.005;
+    c->dt_max = 0.0;
+    c->max_steps = 0;
+    c->flux_precision = FVG_FLUX_FP64;
+}
+
+int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* err, size_t n) {
+    if (!c || !key_c || !value_c) return fail(err, n, "null argument");
+    const std::string key = trim(key_c), v = value_c, lv = lower(trim(value_c));
+    double d;
+    int k;
+    auto num = [&](double& dst) {
+        if (!parse_num(v, d)) return fail(err, n, "'" + key + "': not a number: '" + v + "'");
+        dst = d;
+        return int(FVG_OK);
+    };
+    auto integer = [&](int& dst) {
+        if (!parse_int(v, k)) return fail(err, n, "'" + key + "': not an integer: '" + v + "'");
+        dst = k;
+        return int(FVG_OK);
+    };
+    if (key == "nx") return integer(c->nx);
+    if (key == "ny") return integer(c->ny);
+    if (key == "nz") return integer(c->nz);
+    if (key == "Lx") return num(c->Lx);
+    if (key == "Ly") return num(c->Ly);
+    if (key == "Lz") return num(c->Lz);
+    if (key == "gamma") return num(c->gamma);
+    if (key == "p_inf") return num(c->p_inf);
+    if (key == "u_in") return num(c->u_in);
+    if (key == "v_in") return num(c->v_in);
+    if (key == "rho_in") return num(c->rho_in);
+    if (key == "p_out") return num(c->p_out);
+    if (key == "shock_blend") return num(c->shock_blend);
+    if (key == "weno_epsilon") return num(c->weno_epsilon);
+    if (key == "boundary_order") return integer(c->boundary_order);
+    if (key == "cfl") return num(c->cfl);
+    if (key == "mu") return num(c->mu);
+    if (key == "pr") return num(c->pr);
+    if (key == "t_end") return num(c->t_end);
+    if (key == "dt_max") return num(c->dt_max);
+    if (key == "max_steps") return integer(c->max_steps);
+    if (key == "use_viscous") {
+        if (!parse_flag(v, k)) return fail(err, n, "'use_viscous': not a flag: '" + v + "'");
+        c->use_viscous = k;
+        return FVG_OK;
+    }
+    if (key == "eos_type") {   // SimParams::from: "ideal gas", anything else the stiffened gas (Simulation.cpp:232-235)
+        c->eos_type = lv == "ideal gas" ? FVG_EOS_IDEAL : FVG_EOS_STIFFENED;
+        return FVG_OK;
+    }
+    if (key == "riemann_solver") {   // parse_riemann (Schemes.cpp:77-85)
+        if (lv == "hllc") return (c->riemann = FVG_HLLC), FVG_OK;
+        if (lv == "hllc (low-mach thornber)" || lv == "lowmach" || lv == "lowmachhllc" || lv == "low-mach")
+            return (c->riemann = FVG_LOWMACH_HLLC), FVG_OK;
+        if (lv == "hllc (preconditioned)" || lv == "preconditioned" || lv == "preconditionedhllc")
+            return fail(err, n, "The preconditioned Riemann solver was removed: use 'HLLC (Low-Mach Thornber)' (or 'HLLC')");
+        return fail(err, n, "Unknown Riemann solver: '" + v + "'");
+    }
+    if (key == "muscl_limiter") {   // parse_limiter (Schemes.cpp:87-94)
+        if (lv == "minmod") return (c->limiter = FVG_LIM_MINMOD), FVG_OK;
+        if (lv == "van leer" || lv == "vanleer") return (c->limiter = FVG_LIM_VAN_LEER), FVG_OK;
+        if (lv == "mc") return (c->limiter = FVG_LIM_MC), FVG_OK;
+        if (lv == "koren") return (c->limiter = FVG_LIM_KOREN), FVG_OK;
+        return fail(err, n, "Unknown MUSCL limiter: '" + v + "' (use Minmod, Van Leer, MC or Koren)");
+    }
+    if (key == "reconstruction") {   // parse_reconstruction (Schemes.cpp:96-102) plus "none"
+        if (lv == "none") return (c->reconstruction = FVG_RECON_NONE), FVG_OK;
+        if (lv == "muscl") return (c->reconstruction = FVG_RECON_MUSCL), FVG_OK;
+        if (lv == "weno3") return (c->reconstruction = FVG_RECON_WENO3), FVG_OK;
+        if (lv == "weno5") return (c->reconstruction = FVG_RECON_WENO5), FVG_OK;
+        return fail(err, n, "Unknown reconstruction: '" + v + "' (use muscl, weno3 or weno5)");
+    }
+    if (key == "time_scheme") {   // parse_time_scheme (TimeScheme.cpp:78-85) plus "auto"
+        if (lv == "auto") return (c->time_scheme = FVG_TS_AUTO), FVG_OK;
+        if (lv == "explicit euler" || lv == "euler" || lv == "explicit_euler") return (c->time_scheme = FVG_TS_EULER), FVG_OK;
+        if (lv == "ssp runge-kutta 2" || lv == "ssprk2" || lv == "rk2") return (c->time_scheme = FVG_TS_SSPRK2), FVG_OK;
+        if (lv == "ssp runge-kutta 3" || lv == "ssprk3" || lv == "rk3") return (c->time_scheme = FVG_TS_SSPRK3), FVG_OK;
+        if (lv == "muscl-hancock" || lv == "hancock") return (c->time_scheme = FVG_TS_HANCOCK), FVG_OK;
+        return fail(err, n, "Unknown time scheme: '" + v + "'");
+    }
+    if (key == "flux_precision") {
+        if (lv == "fp64" || lv == "double") return (c->flux_precision = FVG_FLUX_FP64), FVG_OK;
+        if (lv == "fp32" || lv == "float") return (c->flux_precision = FVG_FLUX_FP32), FVG_OK;
+        return fail(err, n, "Unknown flux precision: '" + v + "' (use fp64 or fp32)");
+    }
+    static const char* bc_keys[6] = {"bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax", "bc_zmin", "bc_zmax"};
+    for (int s = 0; s < 6; ++s) {
+        if (key != bc_keys[s]) continue;
+        // parse_bc (Boundaries.cpp:15-27)
+        int b = -1;
+        if (lv == "wall" || lv == "inviscid wall (slip)") b = FVG_BC_WALL;
+        else if (lv == "no_slip" || lv == "viscous wall (no-slip)") b = FVG_BC_NO_SLIP;
+        else if (lv == "moving_wall" || lv == "moving wall") b = FVG_BC_MOVING_WALL;
+        else if (lv == "vel_inlet" || lv == "velocity inlet") b = FVG_BC_VEL_INLET;
+        else if (lv == "massflow_inlet" || lv == "mass flow inlet") b = FVG_BC_MASSFLOW_INLET;
+        else if (lv == "outlet" || lv == "pressure outlet") b = FVG_BC_OUTLET;
+        else if (lv == "transmissive") b = FVG_BC_TRANSMISSIVE;
+        else if (lv == "periodic") b = FVG_BC_PERIODIC;
+        else if (lv == "nonreflecting" || lv == "non_reflecting" || lv == "non-reflecting outlet") b = FVG_BC_NONREFLECTING;
+        if (b < 0) return fail(err, n, "Unknown boundary condition type: '" + v + "'");
+        c->bc[s] = b;
+        return FVG_OK;
+    }
+    return fail(err, n, "Unknown key: '" + key + "'");
+}
+
+int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
+    if (c->time_scheme == FVG_TS_AUTO)   // SimParams::from (Params.cpp:360-363)
+        c->time_scheme = c->reconstruction != FVG_RECON_MUSCL ? FVG_TS_EULER : (c->use_viscous ? FVG_TS_SSPRK2 : FVG_TS_HANCOCK);
+    if (c->nx <= 0 || c->ny <= 0 || c->nz <= 0) return fail(err, n, "nx, ny and nz must be positive");
+    if (!(c->Lx > 0) || !(c->Ly > 0) || !(c->Lz > 0)) return fail(err, n, "Lx, Ly and Lz must be positive");
+    for (int a = 0; a < 3; ++a) {
+        const bool lo = c->bc[2 * a] == FVG_BC_PERIODIC, hi = c->bc[2 * a + 1] == FVG_BC_PERIODIC;
+        if (a < 2 || c->nz > 1)
+            if (lo != hi) return fail(err, n, "Periodic boundaries come in pairs");
+    }
+    if (c->boundary_order != 1 && c->boundary_order != 2) return fail(err, n, "boundary_order must be 1 or 2");
+    return FVG_OK;
+}
+
+double fvg_dt_length(const fvg_config* c) {
+    const double dx = c->Lx / c->nx, dy = c->Ly / c->ny;
+    if (c->nz > 1) {
+        const double dz = c->Lz / c->nz;
+        return 2.0 / (1.0 / dx + 1.0 / dy + 1.0 / dz);
+    }
+    return (dy < dx) ? dy : dx;   // std::min(dx, dy)
+}
+
+}   // extern "C"

@@ -1,0 +1,787 @@
+// Kernels of the uniform-grid residual and time step; see fvg_kernels.hpp for the pass structure.
+// Reference lines are cited per kernel; the arithmetic itself is in fvg_device.hpp.
+#include "fvg_kernels.hpp"
+
+namespace fvg {
+
+constexpr double MU_DIAG = 4.0 / 3.0;   // Physics.cpp:488
+constexpr double MU_OFF = 2.0 / 3.0;
+
+__device__ __forceinline__ long long cidx(const KP& p, int i, int j, int k) { return ((long long)k * p.ny + j) * p.nx + i; }
+__device__ __forceinline__ int ncell(const KP& p, int d) { return d == 0 ? p.nx : (d == 1 ? p.ny : p.nz); }
+__device__ __forceinline__ double hstep(const KP& p, int d) { return d == 0 ? p.dx : (d == 1 ? p.dy : p.dz); }
+// the coordinate along d of (i, j, k), and (i, j, k) with that coordinate replaced by t
+__device__ __forceinline__ int coord(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+__device__ __forceinline__ long long cidx_d(const KP& p, int d, int i, int j, int k, int t) {
+    return cidx(p, d == 0 ? t : i, d == 1 ? t : j, d == 2 ? t : k);
+}
+// Accessors of the per-side / per-axis parameters without run-time array indexing (which would move the
+// kernel arguments to scratch memory).
+__device__ __forceinline__ int halo_of(const KP& p, int s) {
+    switch (s) {
+    case 0: return p.halo[0];
+    case 1: return p.halo[1];
+    case 2: return p.halo[2];
+    case 3: return p.halo[3];
+    case 4: return p.halo[4];
+    default: return p.halo[5];
+    }
+}
+__device__ __forceinline__ int vper_of(const KP& p, int a) { return a == 0 ? p.vper[0] : (a == 1 ? p.vper[1] : p.vper[2]); }
+__device__ __forceinline__ int wrap_of(const KP& p, int a) { return a == 0 ? p.wrap[0] : (a == 1 ? p.wrap[1] : p.wrap[2]); }
+
+template <int NV> __device__ __forceinline__ void load_cons(const KP& p, long long c, double* u) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) u[q] = p.U[q * p.nc + c];
+}
+template <int NV> __device__ __forceinline__ void load_prim(const KP& p, long long c, double* w) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) w[q] = p.W[q * p.nc + c];
+}
+
+// Wave-level then block-level maximum of non-negative, non-NaN doubles, then one atomicMax of the bit pattern.
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double other = __shfl_xor(v, o, 64);
+        v = (other > v) ? other : v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; ++w) m = (red[w] > m) ? red[w] : m;
+        if (m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Stencil access
+// ------------------------------------------------------------------------------------------------------------
+
+// The padded primitive state of the reconstruction at the cell whose coordinate along d is t (the other two
+// from (i, j, k)): the cell itself, or outside the domain the periodic wrap, the ghost-slab layer
+// (fill_ghost_slab, Boundaries.cpp:202-228) or the edge cell (pad_primitives, Schemes.cpp:915-969).
+template <int NV>
+__device__ __forceinline__ void stencil_prim(const KP& p, int d, int i, int j, int k, int t, double* w) {
+    const int n = ncell(p, d);
+    if (t >= 0 && t < n) {
+        load_prim<NV>(p, cidx_d(p, d, i, j, k, t), w);
+        return;
+    }
+    const int hi = t >= n, side = 2 * d + hi;
+    const int h = halo_of(p, side);
+    if (h == 1) {
+        load_prim<NV>(p, cidx_d(p, d, i, j, k, hi ? t - n : t + n), w);
+    } else if (h == 2) {
+        const int l = hi ? t - n : -1 - t;
+        double ui[NV], ug[NV];
+        load_cons<NV>(p, cidx_d(p, d, i, j, k, hi ? n - 1 - l : l), ui);
+        ghost_state<NV>(p.rs.eos, p.bp, side, ui, ug);
+        c2p<NV>(p.rs.eos, ug, w, (double*)nullptr);
+    } else {
+        load_prim<NV>(p, cidx_d(p, d, i, j, k, t < 0 ? 0 : n - 1), w);
+    }
+}
+
+// First-order state outside side `side` (along axis d) of the boundary cell (i, j, k) (apply_side,
+// Boundaries.cpp:259-371): the opposite cell on a periodic side, else the ghost state of the cell.
+template <int NV>
+__device__ __forceinline__ void outside_first_order(const KP& p, int d, int side, int i, int j, int k, double* ug) {
+    if (bc_of(p.bp, side) == 7) {
+        load_cons<NV>(p, cidx_d(p, d, i, j, k, (side & 1) ? 0 : ncell(p, d) - 1), ug);
+        return;
+    }
+    double ui[NV];
+    load_cons<NV>(p, cidx(p, i, j, k), ui);
+    ghost_state<NV>(p.rs.eos, p.bp, side, ui, ug);
+}
+
+// MUSCL face values of one variable (Schemes.cpp:627-635): wl from (c0, c1, c2), wr from (c1, c2, c3)
+__device__ __forceinline__ double muscl_wl(int lim, double c0, double c1, double c2) { return c1 + 0.5 * limiter(lim, c1 - c0, c2 - c1); }
+__device__ __forceinline__ double muscl_wr(int lim, double c1, double c2, double c3) { return c2 - 0.5 * limiter(lim, c3 - c2, c2 - c1); }
+
+// ------------------------------------------------------------------------------------------------------------
+// Viscous padded state (Boundaries.cpp:379-700): cell coordinates may lie one cell outside the domain.
+// ------------------------------------------------------------------------------------------------------------
+
+// viscous_ghost_3d (Boundaries.cpp:253-269); the 2-D frame rules are the same on the first four components.
+template <int NV>
+__device__ __forceinline__ void viscous_ghost(const KP& p, int side, const double* c, double* g) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) g[q] = c[q];
+    const int d = side >> 1, bc = bc_of(p.bp, side);
+    if (bc == 1) {   // no-slip
+#pragma unroll
+        for (int a = 0; a < NV - 2; ++a) g[mom(a)] = -c[mom(a)];
+    } else if (bc == 0) {   // slip wall
+#pragma unroll
+        for (int a = 0; a < NV - 2; ++a)
+            if (a == d) g[mom(a)] = -c[mom(a)];
+    } else if (bc == 2) {   // moving wall
+#pragma unroll
+        for (int t = 0; t < NV - 2; ++t) {
+            const double uw = t == 0 ? p.bp.u_in : (t == 1 ? p.bp.v_in : 0.0);
+            g[mom(t)] = (t == d) ? -c[mom(t)] : 2.0 * c[0] * uw - c[mom(t)];
+        }
+        if (p.bo2) {
+            // match_kinetic_energy (Boundaries.cpp:194-199); in 2-D the w terms are +0 and are kept for the same rounding
+            const double g4 = NV == 5 ? g[NV - 1] : 0.0, c4 = NV == 5 ? c[NV - 1] : 0.0;
+            g[3] = c[3] + 0.5 * (g[1] * g[1] + g[2] * g[2] + g4 * g4 - c[1] * c[1] - c[2] * c[2] - c4 * c4) / c[0];
+        }
+    }
+}
+
+// The face-ghost entry of the viscous padded state at cell coordinates (i, j, k) outside the domain along axis
+// a only (padded_3d / get_padded_frame side rules): the periodic opposite cell, the wall rules, else the edge copy.
+template <int NV>
+__device__ __forceinline__ void visc_face_ghost(const KP& p, int i, int j, int k, int a, double* out) {
+    const int n = ncell(p, a);
+    const int hi = coord(a, i, j, k) >= n, side = 2 * a + hi;
+    const int bc = bc_of(p.bp, side);
+    if (bc == 7) {
+        load_cons<NV>(p, cidx_d(p, a, i, j, k, hi ? 0 : n - 1), out);
+        return;
+    }
+    double in[NV];
+    load_cons<NV>(p, cidx_d(p, a, i, j, k, hi ? n - 1 : 0), in);
+    if (bc == 0 || bc == 1 || bc == 2) {
+        viscous_ghost<NV>(p, side, in, out);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = in[q];
+    }
+}
+
+// Conservative padded state at cell coordinates (i, j, k), each in [-1, n]. One axis outside: the side rule;
+// two axes outside with boundary_order = 2: the rule of the lower axis applied to the face-ghost entry of the
+// higher one (fix_padded_corners, Boundaries.cpp:520-560; fix_frame_corners, :562-604); otherwise the edge copy.
+template <int NV>
+__device__ __forceinline__ void visc_pad_cons(const KP& p, int i, int j, int k, double* out) {
+    const int ox = i < 0 || i >= p.nx, oy = j < 0 || j >= p.ny, oz = k < 0 || k >= p.nz;
+    const int nout = ox + oy + oz;
+    if (nout == 0) {
+        load_cons<NV>(p, cidx(p, i, j, k), out);
+        return;
+    }
+    if (nout == 1) {
+        visc_face_ghost<NV>(p, i, j, k, ox ? 0 : (oy ? 1 : 2), out);
+        return;
+    }
+    if (nout == 2 && p.bo2) {
+        const int d = ox ? 0 : 1, second = (ox && oy) ? 1 : 2;
+        const int n = ncell(p, d);
+        const int hi = coord(d, i, j, k) >= n, side = 2 * d + hi;
+        if (bc_of(p.bp, side) == 7) {
+            const int t = hi ? 0 : n - 1;
+            visc_face_ghost<NV>(p, d == 0 ? t : i, d == 1 ? t : j, k, second, out);
+        } else {
+            const int t = hi ? n - 1 : 0;
+            double cin[NV];
+            visc_face_ghost<NV>(p, d == 0 ? t : i, d == 1 ? t : j, k, second, cin);
+            viscous_ghost<NV>(p, side, cin, out);
+        }
+        return;
+    }
+    const int ci = i < 0 ? 0 : (i >= p.nx ? p.nx - 1 : i);
+    const int cj = j < 0 ? 0 : (j >= p.ny ? p.ny - 1 : j);
+    const int ck = k < 0 ? 0 : (k >= p.nz ? p.nz - 1 : k);
+    load_cons<NV>(p, cidx(p, ci, cj, ck), out);
+}
+
+template <int NV>
+__device__ __forceinline__ void visc_pad_prim(const KP& p, int i, int j, int k, double* w) {
+    double u[NV];
+    visc_pad_cons<NV>(p, i, j, k, u);
+    c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_cells: primitives; with viscosity the cell velocity gradients and the viscous speed bound
+// (Physics.cpp:44-76 2-D, 520-555 and 733-741 3-D; ViscousBound.hpp:16-24)
+// ------------------------------------------------------------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(256) void k_cells(KP p) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double local = 0.0;
+    if (c < p.nc && !p.st->done) {
+        double u[NV], w[NV];
+        load_cons<NV>(p, c, u);
+        c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) p.W[q * p.nc + c] = w[q];
+        if (p.viscous) {
+            const int i = (int)(c % p.nx);
+            const long long r = c / p.nx;
+            const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+            const double rho = w[0];
+            constexpr int ND = NV - 2;
+            double grad[ND][ND + 1];   // grad[b][slot]: slots 0 (rho) and the momenta 1..ND
+#pragma unroll
+            for (int b = 0; b < ND; ++b) {
+                double lo[NV], hi[NV];
+                const int o0[3] = {b == 0 ? -1 : 0, b == 1 ? -1 : 0, b == 2 ? -1 : 0};
+                visc_pad_cons<NV>(p, i + o0[0], j + o0[1], k + o0[2], lo);
+                visc_pad_cons<NV>(p, i - o0[0], j - o0[1], k - o0[2], hi);
+                const double h2 = 2.0 * hstep(p, b);
+                grad[b][0] = (hi[0] - lo[0]) / h2;
+#pragma unroll
+                for (int a = 0; a < ND; ++a) grad[b][a + 1] = (hi[mom(a)] - lo[mom(a)]) / h2;
+            }
+#pragma unroll
+            for (int a = 0; a < ND; ++a) {
+                const double ua = w[mom(a)];
+#pragma unroll
+                for (int b = 0; b < ND; ++b) p.G[(a * ND + b) * p.nc + c] = (grad[b][a + 1] - ua * grad[b][0]) / rho;
+            }
+            const double g = p.rs.eos.gamma;
+            const double nu = smax((4.0 / 3.0) * (p.mu / rho), (p.k_cond * (g - 1.0)) / rho) + 1e-12;
+            double mv = fabs(w[1]);
+            if (mv < fabs(w[2])) mv = fabs(w[2]);
+            if constexpr (NV == 5)
+                if (mv < fabs(w[4])) mv = fabs(w[4]);
+            const double s = mv + sound_speed(p.rs.eos, w[3], rho) + 2.0 * p.Ldt * p.inv_h2 * nu;
+            local = smax(local, s);
+        }
+    }
+    if (p.viscous) block_max_to(local, &p.st->s_visc);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_hancock: the limited states of every cell advanced by half a step with the cell's own flux difference;
+// an inadmissible state stays as it was (Schemes.cpp:2135-2199, 2907-2933, 2989-3016).
+// ------------------------------------------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void euler_flux_axis(const Eos& e, const double* U, int a, double* F) {
+    double W[NV];
+    c2p<NV>(e, U, W, (double*)nullptr);
+    const double un = W[mom(a)], pr = W[3];
+    F[0] = U[0] * un;
+    F[1] = U[1] * un + (a == 0 ? pr : 0.0);
+    F[2] = U[2] * un + (a == 1 ? pr : 0.0);
+    F[3] = (U[3] + pr) * un;
+    if constexpr (NV == 5) F[4] = U[4] * un + (a == 2 ? pr : 0.0);
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_hancock(KP p) {
+    constexpr int ND = NV - 2, NS = 2 * ND;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long reverted = 0;
+    if (c < p.nc && !p.st->done) {
+        const int i = (int)(c % p.nx);
+        const long long r = c / p.nx;
+        const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+        double st[NS][NV];
+        double w0[NV];
+        load_prim<NV>(p, c, w0);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int t = coord(d, i, j, k), n = ncell(p, d);
+            const bool up_lo = t > 0 || halo_of(p, 2 * d);          // the cell's low face is reconstructed
+            const bool up_hi = t + 1 < n || halo_of(p, 2 * d + 1);  // its high face
+            double wm[NV], wp[NV];
+            stencil_prim<NV>(p, d, i, j, k, t - 1, wm);
+            stencil_prim<NV>(p, d, i, j, k, t + 1, wp);
+            double lo[NV], hi[NV], uc[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                lo[q] = muscl_wr(p.lim, wm[q], w0[q], wp[q]);
+                hi[q] = muscl_wl(p.lim, wm[q], w0[q], wp[q]);
+            }
+            load_cons<NV>(p, c, uc);
+            roundtrip<NV>(p.rs.eos, uc);   // the first-order state of a side face without a halo
+            double a0[NV], a1[NV];
+            p2c<NV>(p.rs.eos, lo, a0);
+            p2c<NV>(p.rs.eos, hi, a1);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                st[2 * d][q] = up_lo ? a0[q] : uc[q];
+                st[2 * d + 1][q] = up_hi ? a1[q] : uc[q];
+            }
+        }
+        const double half_dt = p.st->half_dt;
+        const double hx = half_dt / p.dx, hy = half_dt / p.dy;
+        double f[NS][NV];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) euler_flux_axis<NV>(p.rs.eos, st[s], s / 2, f[s]);
+        double dd[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            if constexpr (NV == 5) {
+                const double hz = half_dt / p.dz;
+                dd[q] = (hx * (f[1][q] - f[0][q]) + hy * (f[3][q] - f[2][q])) + hz * (f[5][q] - f[4][q]);
+            } else {
+                dd[q] = hx * (f[1][q] - f[0][q]) + hy * (f[3][q] - f[2][q]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double nw[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) nw[q] = st[s][q] - dd[q];
+            const double u4 = NV == 5 ? nw[NV - 1] : 0.0;
+            // admissible_state (Schemes.cpp:2145-2148)
+            const bool ok = (nw[0] > kEosEps) && (0.5 * (nw[1] * nw[1] + nw[2] * nw[2] + u4 * u4) < nw[0] * (nw[3] - p.rs.eos.p_inf));
+            reverted += ok ? 0 : 1;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) p.HS[((long long)s * NV + q) * p.nc + c] = ok ? nw[q] : st[s][q];
+        }
+    }
+    if (reverted) atomicAdd((unsigned long long*)&p.st->reverted, reverted);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_faces<axis>: the faces normal to one axis, one thread per face: face states (reconstruction, first order,
+// or the MUSCL-Hancock predicted states), the Riemann flux, and the viscous flux of the face.
+// Schemes.cpp:389-700 (states), 842-911 (fused line), 231-240 (Riemann); Physics.cpp:100-176 (2-D),
+// 567-707 (3-D) for the viscous flux.
+// ------------------------------------------------------------------------------------------------------------
+// Cross derivative d u_a / d x_b on a face normal to D: the average of the cell gradients of the two cells xL, xR
+// (the opposite cell across a periodic seam), or on a physical boundary face with boundary_order = 2 the
+// tangential difference of the face values, ghost and cell layer averaged (Physics.cpp:614-628).
+template <int NV, int D>
+__device__ __forceinline__ double cross_term(const KP& p, bool phys, int i, int j, int k, int tg, int tc, long long xL, long long xR, int a,
+                                             int b) {
+    constexpr int ND = NV - 2;
+    if (phys) {
+        double f[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int off = s == 0 ? 1 : -1;
+            const int oi = b == 0 ? off : 0, oj = b == 1 ? off : 0, ok = b == 2 ? off : 0;
+            double wg[NV], wc[NV];
+            visc_pad_prim<NV>(p, (D == 0 ? tg : i) + oi, (D == 1 ? tg : j) + oj, (D == 2 ? tg : k) + ok, wg);
+            visc_pad_prim<NV>(p, (D == 0 ? tc : i) + oi, (D == 1 ? tc : j) + oj, (D == 2 ? tc : k) + ok, wc);
+            f[s] = 0.5 * (wg[mom(a)] + wc[mom(a)]);
+        }
+        return (f[0] - f[1]) / (2.0 * hstep(p, b));
+    }
+    const double* Gp = p.G + (long long)(a * ND + b) * p.nc;
+    return 0.5 * (Gp[xL] + Gp[xR]);
+}
+
+template <int NV, int D>
+__device__ __forceinline__ void viscous_face(const KP& p, int i, int j, int k, int t, int n, double* out) {
+    constexpr int ND = NV - 2;
+    const bool lo = t == 0, hi = t == n;
+    const long long cL = cidx_d(p, D, i, j, k, lo ? 0 : t - 1), cR = cidx_d(p, D, i, j, k, hi ? n - 1 : t);
+    double vL[ND], vR[ND], TL, TR;
+    {
+        double w[NV];
+        if (lo) visc_pad_prim<NV>(p, D == 0 ? -1 : i, D == 1 ? -1 : j, D == 2 ? -1 : k, w);
+        else load_prim<NV>(p, cL, w);
+#pragma unroll
+        for (int a = 0; a < ND; ++a) vL[a] = w[mom(a)];
+        TL = w[3] / w[0];
+        if (hi) visc_pad_prim<NV>(p, D == 0 ? n : i, D == 1 ? n : j, D == 2 ? n : k, w);
+        else load_prim<NV>(p, cR, w);
+#pragma unroll
+        for (int a = 0; a < ND; ++a) vR[a] = w[mom(a)];
+        TR = w[3] / w[0];
+    }
+    const long long xL = (lo && vper_of(p, D)) ? cidx_d(p, D, i, j, k, n - 1) : cL;
+    const long long xR = (hi && vper_of(p, D)) ? cidx_d(p, D, i, j, k, 0) : cR;
+    const bool phys = p.bo2 && !vper_of(p, D) && (lo || hi);
+    const double h = hstep(p, D);
+    const int tg = lo ? -1 : n, tc = lo ? 0 : n - 1;   // the ghost and the cell layer of a boundary face
+#define CROSS(a, b) cross_term<NV, D>(p, phys, i, j, k, tg, tc, xL, xR, a, b)
+    const double mu = p.mu;
+    if constexpr (ND == 2) {
+        const double u_f = 0.5 * (vL[0] + vR[0]), v_f = 0.5 * (vL[1] + vR[1]);
+        if constexpr (D == 0) {
+            const double du_dx = (vR[0] - vL[0]) / h, dv_dx = (vR[1] - vL[1]) / h, dT_dx = (TR - TL) / h;
+            const double du_dy = CROSS(0, 1), dv_dy = CROSS(1, 1);
+            const double tau_xx = mu * (MU_DIAG * du_dx - MU_OFF * dv_dy);
+            const double tau_xy = mu * (du_dy + dv_dx);
+            const double q_x = -p.k_cond * dT_dx;
+            out[0] = tau_xx * p.dy;
+            out[1] = tau_xy * p.dy;
+            out[2] = (u_f * tau_xx + v_f * tau_xy - q_x) * p.dy;
+        } else {
+            const double dv_dy = (vR[1] - vL[1]) / h, du_dy = (vR[0] - vL[0]) / h, dT_dy = (TR - TL) / h;
+            const double du_dx = CROSS(0, 0), dv_dx = CROSS(1, 0);
+            const double tau_yy = mu * (MU_DIAG * dv_dy - MU_OFF * du_dx);
+            const double tau_yx = mu * (du_dy + dv_dx);
+            const double q_y = -p.k_cond * dT_dy;
+            out[0] = tau_yx * p.dx;
+            out[1] = tau_yy * p.dx;
+            out[2] = (u_f * tau_yx + v_f * tau_yy - q_y) * p.dx;
+        }
+    } else {
+        double Gn[3], vf[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            Gn[a] = (vR[a] - vL[a]) / h;
+            vf[a] = 0.5 * (vL[a] + vR[a]);
+        }
+        const double dT = (TR - TL) / h;
+        double tau[3];
+        if constexpr (D == 0) {
+            const double g01 = CROSS(0, 1), g11 = CROSS(1, 1), g22 = CROSS(2, 2), g02 = CROSS(0, 2);
+            tau[0] = mu * (MU_DIAG * Gn[0] - MU_OFF * ((0.0 + g11) + g22));
+            tau[1] = mu * (Gn[1] + g01);
+            tau[2] = mu * (Gn[2] + g02);
+        } else if constexpr (D == 1) {
+            const double g00 = CROSS(0, 0), g10 = CROSS(1, 0), g22 = CROSS(2, 2), g12 = CROSS(1, 2);
+            tau[0] = mu * (Gn[0] + g10);
+            tau[1] = mu * (MU_DIAG * Gn[1] - MU_OFF * ((0.0 + g00) + g22));
+            tau[2] = mu * (Gn[2] + g12);
+        } else {
+            const double g00 = CROSS(0, 0), g11 = CROSS(1, 1), g20 = CROSS(2, 0), g21 = CROSS(2, 1);
+            tau[0] = mu * (Gn[0] + g20);
+            tau[1] = mu * (Gn[1] + g21);
+            tau[2] = mu * (MU_DIAG * Gn[2] - MU_OFF * ((0.0 + g00) + g11));
+        }
+        const double q_d = -p.k_cond * dT;
+        const double area = D == 0 ? p.Ax : (D == 1 ? p.Ay : p.Az);
+        out[0] = tau[0] * area;
+        out[1] = tau[1] * area;
+        out[2] = (vf[0] * tau[0] + vf[1] * tau[1] + vf[2] * tau[2] - q_d) * area;
+        out[3] = tau[2] * area;
+    }
+}
+
+#undef CROSS
+
+// The MUSCL-Hancock state outside a boundary face (side `side`, boundary cell (i, j, k) whose predicted state
+// on the face is `in`): the seam copy of the cell across a periodic wrap, the reconstructed state of the ghost
+// layers, or the first-order state; on a wall side the ghost of the predicted state (hancock_wall_ghosts).
+template <int NV, int D>
+__device__ __forceinline__ void hancock_outside(const KP& p, int side, int i, int j, int k, int t, const double* in, double* out) {
+    const int n = ncell(p, D), hi = side & 1;
+    if (wrap_of(p, D)) {
+        const long long cc = cidx_d(p, D, i, j, k, hi ? 0 : n - 1);
+        const int s = hi ? 2 * D : 2 * D + 1;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = p.HS[((long long)s * NV + q) * p.nc + cc];
+    } else if (halo_of(p, side)) {
+        double w[4][NV], v[NV];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) stencil_prim<NV>(p, D, i, j, k, t - 2 + s, w[s]);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = hi ? muscl_wr(p.lim, w[1][q], w[2][q], w[3][q]) : muscl_wl(p.lim, w[0][q], w[1][q], w[2][q]);
+        p2c<NV>(p.rs.eos, v, out);
+    } else {
+        outside_first_order<NV>(p, D, side, i, j, k, out);
+        roundtrip<NV>(p.rs.eos, out);
+    }
+    const int bc = bc_of(p.bp, side);
+    if (bc == 0 || bc == 1 || bc == 2) ghost_state<NV>(p.rs.eos, p.bp, side, in, out);
+}
+
+template <int NV, int D, int MODE>
+__global__ __launch_bounds__(256) void k_faces(KP p) {
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
+    const long long nf = (long long)ni * nj * nk;
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double local = 0.0;
+    if (f < nf && !p.st->done) {
+        const int i = (int)(f % ni);
+        const long long r = f / ni;
+        const int j = (int)(r % nj), k = (int)(r / nj);
+        const int t = coord(D, i, j, k);
+        const int n = ncell(p, D);
+        const bool lo = t == 0, hi = t == n;
+        // the cells left and right of the face (their coordinate along D is t - 1 and t)
+        const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
+        double UL[NV], UR[NV];
+        if constexpr (MODE == MODE_FIRST_ORDER) {
+            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), UL);
+            else outside_first_order<NV>(p, D, 2 * D, i, j, k, UL);
+            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), UR);
+            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, UR);
+        } else if constexpr (MODE == MODE_PLAIN) {
+            const bool upgrade = (t > 0 && t < n) || (lo && halo_of(p, 2 * D)) || (hi && halo_of(p, 2 * D + 1));
+            if (upgrade) {
+                double wl[NV], wr[NV];
+                if (p.recon == 3) {   // WENO5-Z on six cells t-3 .. t+2
+                    double c[6][NV];
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) stencil_prim<NV>(p, D, i, j, k, t - 3 + s, c[s]);
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], p.weno_eps);
+                        wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], p.weno_eps);
+                    }
+                } else {   // four cells t-2 .. t+1
+                    double c[4][NV];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) stencil_prim<NV>(p, D, i, j, k, t - 2 + s, c[s]);
+                    if (p.recon == 2) {
+#pragma unroll
+                        for (int q = 0; q < NV; ++q) {
+                            wl[q] = weno3z(c[0][q], c[1][q], c[2][q], p.weno_eps);
+                            wr[q] = weno3z(c[3][q], c[2][q], c[1][q], p.weno_eps);
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NV; ++q) {
+                            wl[q] = muscl_wl(p.lim, c[0][q], c[1][q], c[2][q]);
+                            wr[q] = muscl_wr(p.lim, c[1][q], c[2][q], c[3][q]);
+                        }
+                    }
+                }
+                p2c<NV>(p.rs.eos, wl, UL);
+                p2c<NV>(p.rs.eos, wr, UR);
+            } else {
+                if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), UL);
+                else outside_first_order<NV>(p, D, 2 * D, i, j, k, UL);
+                if (!hi) load_cons<NV>(p, cidx(p, i, j, k), UR);
+                else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, UR);
+                roundtrip<NV>(p.rs.eos, UL);
+                roundtrip<NV>(p.rs.eos, UR);
+            }
+        } else {   // MUSCL-Hancock: the predicted states of the cells on both sides (Schemes.cpp:2934-2988)
+            if (!lo) {
+                const long long cc = cidx(p, il, jl, kl);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) UL[q] = p.HS[((long long)(2 * D + 1) * NV + q) * p.nc + cc];
+            }
+            if (!hi) {
+                const long long cc = cidx(p, i, j, k);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) UR[q] = p.HS[((long long)(2 * D) * NV + q) * p.nc + cc];
+            }
+            if (lo) hancock_outside<NV, D>(p, 2 * D, i, j, k, t, UR, UL);
+            if (hi) hancock_outside<NV, D>(p, 2 * D + 1, il, jl, kl, t, UL, UR);
+        }
+        double F[NV], sm;
+        face_flux<NV>(p.rs, UL, UR, D, F, sm);
+        if (sm > local) local = sm;
+        double* Fc = p.Fc[D];
+        const long long ps = p.nfa[D];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
+        if (p.viscous) {
+            double v[NV - 1];
+            viscous_face<NV, D>(p, i, j, k, t, n, v);
+            double* Fv = p.Fv[D];
+#pragma unroll
+            for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
+        }
+    }
+    block_max_to(local, &p.st->s_conv);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_update: divergence of the face fluxes (Simulation.cpp:1022-1037, 1293-1310), the viscous balance added
+// (Physics.cpp:196-204, 746-752) and the stage combination of the time scheme (TimeScheme.cpp:27-59, 150-156,
+// 213-219), one cell per thread.
+// ------------------------------------------------------------------------------------------------------------
+template <int NV, int UPD>
+__global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nc || p.st->done) return;
+    const int i = (int)(c % p.nx);
+    const long long r = c / p.nx;
+    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+    const long long fx = ((long long)k * p.ny + j) * (p.nx + 1) + i;
+    const long long fy = ((long long)k * (p.ny + 1) + j) * p.nx + i;
+    const long long fz = c;
+    const long long fyt = fy + p.nx, fzk = fz + (long long)p.nx * p.ny;
+    double R[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const double* Fx = p.Fc[0] + q * p.nfa[0];
+        const double* Fy = p.Fc[1] + q * p.nfa[1];
+        const double fl = Fx[fx], fr = Fx[fx + 1], fb = Fy[fy], ft = Fy[fyt];
+        if constexpr (NV == 5) {
+            const double* Fz = p.Fc[2] + q * p.nfa[2];
+            const double ff = Fz[fz], fk = Fz[fzk];
+            R[q] = (((0.0 - (fr * p.Ax - fl * p.Ax)) - (ft * p.Ay - fb * p.Ay)) - (fk * p.Az - ff * p.Az)) * p.inv_vol;
+        } else {
+            R[q] = ((0.0 - (fr * p.Ax - fl * p.Ax)) - (ft * p.Ay - fb * p.Ay)) * p.inv_vol;
+        }
+    }
+    if (p.viscous) {
+        R[0] = R[0] + 0.0;   // the zero viscous flux of the density slot
+#pragma unroll
+        for (int q = 1; q < NV; ++q) {
+            const double* Fx = p.Fv[0] + (q - 1) * p.nfa[0];
+            const double* Fy = p.Fv[1] + (q - 1) * p.nfa[1];
+            const double fl = Fx[fx], fr = Fx[fx + 1], fb = Fy[fy], ft = Fy[fyt];
+            double v;
+            if constexpr (NV == 5) {
+                const double* Fz = p.Fv[2] + (q - 1) * p.nfa[2];
+                v = (((0.0 + (fr - fl)) + (ft - fb)) + (Fz[fzk] - Fz[fz])) * p.inv_vol;
+            } else {
+                v = ((0.0 + (fr - fl)) + (ft - fb)) * p.inv_vol;
+            }
+            R[q] = R[q] + v;
+        }
+    }
+    const double dt = p.st->dt;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const long long x = q * p.nc + c;
+        double o;
+        if constexpr (UPD == UPD_RESIDUAL) o = R[q];
+        else if constexpr (UPD == UPD_AXPY) o = U0[x] + dt * R[q];
+        else if constexpr (UPD == UPD_RK2_2) o = 0.5 * U0[x] + 0.5 * (Ut[x] + dt * R[q]);
+        else if constexpr (UPD == UPD_RK3_2) o = 0.75 * U0[x] + 0.25 * (Ut[x] + dt * R[q]);
+        else o = (1.0 / 3.0) * U0[x] + (2.0 / 3.0) * (Ut[x] + dt * R[q]);
+        out[x] = o;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_control: the scalar part of the time scheme on the device (TimeScheme.cpp:110-319; the divergence check of
+// SimulationInternal.hpp:259-267; the loop condition of SimulationRun.cpp:167).
+// ------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double stage_speed(DevState* s, int viscous) {
+    const double sc = __longlong_as_double((long long)s->s_conv);
+    if (!viscous) return sc;
+    const double sv = __longlong_as_double((long long)s->s_visc);
+    return smax(sc, sv);
+}
+__device__ __forceinline__ double clamp_dt(const DevState* s, double dt) {
+    dt = smin(dt, s->t_max - s->t);
+    if (s->dt_max > 0.0) dt = smin(dt, s->dt_max);
+    return dt;
+}
+__device__ __forceinline__ void end_step(DevState* s) {
+    if (isfinite(s->s_step) && isfinite(s->dt) && s->t + s->dt > s->t) {
+        s->t += s->dt;
+        s->steps += 1;
+    } else {
+        s->diverged = 1;
+        s->done = 1;
+        s->steps += 1;
+    }
+}
+
+// op: the CTL_* step; arg: the time scheme (CTL_STAGE1_DT, CTL_END_RK) or the stage number (CTL_STAGE_S).
+__global__ void k_control(KP p, int op, int arg) {
+    const int scheme = arg;
+    DevState* s = p.st;
+    if (op == CTL_BEGIN_STEP) {
+        if (s->done) return;
+        if (!(s->t < s->t_end) || (s->max_steps > 0 && s->steps >= s->max_steps)) {
+            s->done = 1;
+            return;
+        }
+        s->t_max = s->t_end < s->t_max ? s->t_end : s->t_max;
+        s->s_conv = 0;
+        s->s_visc = 0;
+        return;
+    }
+    if (s->done) return;
+    const double S = stage_speed(s, p.viscous);
+    s->s_conv = 0;
+    s->s_visc = 0;
+    switch (op) {
+    case CTL_STAGE1_DT:
+        s->s1 = S;
+        s->dt = clamp_dt(s, p.cfl * p.Ldt / (S + kDtEps));
+        if (scheme == 0) {   // explicit Euler: the step is one stage
+            s->s_step = S;
+        }
+        break;
+    case CTL_STAGE_S:
+        if (arg == 2) s->s2 = S;
+        else s->s3 = S;
+        break;
+    case CTL_END_RK:
+        if (scheme == 1) s->s_step = smax(s->s1, s->s2);
+        else if (scheme == 2) s->s_step = smax(s->s1, smax(s->s2, s->s3));
+        end_step(s);
+        break;
+    case CTL_SEED: s->lagged = S; break;
+    case CTL_HANCOCK_DT:
+        s->dt = clamp_dt(s, p.cfl * p.Ldt / (s->lagged + kDtEps));
+        s->half_dt = 0.5 * s->dt;
+        break;
+    case CTL_END_HANCOCK:
+        s->lagged = S;
+        s->s_step = S;
+        s->half_dt = 0.0;
+        end_step(s);
+        break;
+    case CTL_RESIDUAL_S: s->s_step = S; break;
+    default: break;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Conserved totals: a fixed-order FP64 tree (per-block partial sums, then one block over the partials).
+// ------------------------------------------------------------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(256) void k_totals_partial(const double* __restrict__ U, long long nc, double* __restrict__ part) {
+    __shared__ double sh[3][256];
+    const long long per = (nc + gridDim.x - 1) / gridDim.x;
+    const long long b0 = blockIdx.x * per, b1 = b0 + per < nc ? b0 + per : nc;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (long long c = b0 + threadIdx.x; c < b1; c += blockDim.x) {
+        const double r = U[c], m1 = U[nc + c], m2 = U[2 * nc + c], E = U[3 * nc + c];
+        const double m4 = NV == 5 ? U[(NV - 1) * nc + c] : 0.0;
+        a0 += r;
+        a1 += 0.5 * (m1 * m1 + m2 * m2 + m4 * m4) / r;
+        a2 += E;
+    }
+    sh[0][threadIdx.x] = a0;
+    sh[1][threadIdx.x] = a1;
+    sh[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[3 * blockIdx.x + q] = sh[q][0];
+}
+__global__ void k_totals_final(const double* __restrict__ part, int nb, long long nc, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 3; ++q) s[q] += part[3 * b + q];
+    for (int q = 0; q < 3; ++q) out[q] = s[q] / (double)nc;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Known-answer kernel: the Riemann flux of independent faces (AoS, 5 doubles per state)
+// ------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_riemann_batch(RiemannParams rs, const double* __restrict__ UL, const double* __restrict__ UR,
+                                                       const int* __restrict__ axis, long long n, double* __restrict__ F,
+                                                       double* __restrict__ smax_out) {
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    double ul[5], ur[5], fo[5], sm;
+    for (int q = 0; q < 5; ++q) {
+        ul[q] = UL[5 * f + q];
+        ur[q] = UR[5 * f + q];
+    }
+    face_flux<5>(rs, ul, ur, axis[f], fo, sm);
+    for (int q = 0; q < 5; ++q) F[5 * f + q] = fo[q];
+    smax_out[f] = sm;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Explicit instantiations used by the host
+// ------------------------------------------------------------------------------------------------------------
+#define FVG_INST(NV)                                                                                                  \
+    template __global__ void k_cells<NV>(KP);                                                                        \
+    template __global__ void k_hancock<NV>(KP);                                                                      \
+    template __global__ void k_faces<NV, 0, MODE_PLAIN>(KP);                                                         \
+    template __global__ void k_faces<NV, 1, MODE_PLAIN>(KP);                                                         \
+    template __global__ void k_faces<NV, 0, MODE_HANCOCK>(KP);                                                       \
+    template __global__ void k_faces<NV, 1, MODE_HANCOCK>(KP);                                                       \
+    template __global__ void k_faces<NV, 0, MODE_FIRST_ORDER>(KP);                                                   \
+    template __global__ void k_faces<NV, 1, MODE_FIRST_ORDER>(KP);                                                   \
+    template __global__ void k_update<NV, UPD_RESIDUAL>(KP, const double*, const double*, double*);                 \
+    template __global__ void k_update<NV, UPD_AXPY>(KP, const double*, const double*, double*);                     \
+    template __global__ void k_update<NV, UPD_RK2_2>(KP, const double*, const double*, double*);                    \
+    template __global__ void k_update<NV, UPD_RK3_2>(KP, const double*, const double*, double*);                    \
+    template __global__ void k_update<NV, UPD_RK3_3>(KP, const double*, const double*, double*);                    \
+    template __global__ void k_totals_partial<NV>(const double*, long long, double*);
+FVG_INST(4)
+FVG_INST(5)
+template __global__ void k_faces<5, 2, MODE_PLAIN>(KP);
+template __global__ void k_faces<5, 2, MODE_HANCOCK>(KP);
+template __global__ void k_faces<5, 2, MODE_FIRST_ORDER>(KP);
+
+}   // namespace fvg

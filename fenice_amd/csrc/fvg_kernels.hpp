@@ -1,0 +1,61 @@
+// Kernels of the uniform-grid residual and time step (gfx950, FP64, structure-of-arrays planes).
+//
+// Layout: a set of planes is one allocation, plane q at base + q * stride. Cells are (k*ny + j)*nx + i.
+// Faces normal to axis d are stored per axis in the reference's face order of that axis
+// (x: ((k*ny + j)*(nx+1) + i), y: ((k*(ny+1) + j)*nx + i), z: ((k*ny + j)*nx + i) with k <= nz).
+//
+// One residual = k_cells (primitives, viscous cell gradients and the viscous speed bound) ->
+// [k_hancock (MUSCL-Hancock only: the predicted face states of every cell)] -> k_faces<axis> per axis
+// (stencil, reconstruction, Riemann solve, viscous face flux; block maximum of the wave speed then one 64-bit
+// atomicMax) -> k_control (time step on the device) -> k_update (flux divergence, viscous balance and the
+// stage combination of the time scheme, fused).
+#pragma once
+
+#include "fvg_device.hpp"
+
+namespace fvg {
+
+// Time-loop state kept on the device, so that a run of steps needs no host round trip.
+struct DevState {
+    unsigned long long s_conv, s_visc;   // bit patterns of the stage's maxima (non-negative doubles order as integers)
+    double t, dt, half_dt, t_end, t_max, dt_max;
+    double s1, s2, s3, s_step, lagged;
+    long long reverted;
+    int done, diverged, steps, max_steps;
+};
+
+struct KP {
+    // mesh
+    int nx, ny, nz, is3d;
+    long long nc;
+    double dx, dy, dz;
+    double Ax, Ay, Az, inv_vol;   // face areas and 1 / volume (2-D: dy, dx, -, 1 / (dx dy))
+    // numerics
+    RiemannParams rs;
+    BcParams bp;
+    int recon, lim, Dh, bo2;
+    double weno_eps;
+    int halo[6];    // 0 none, 1 periodic wrap, 2 ghost slab (boundary_order = 2)
+    int wrap[3];
+    double cfl, Ldt;   // CFL number and dt_length
+    // viscous
+    int viscous;
+    double mu, k_cond, inv_h2;
+    int vper[3];    // periodic axes of the viscous stencil
+    // buffers (plane sets)
+    const double* U;    // stage input state
+    double* W;          // primitives of U
+    double* G;          // viscous cell gradients (3-D: 9 planes G[a][b] at 3a+b; 2-D: dudx, dudy, dvdx, dvdy)
+    double* HS;         // MUSCL-Hancock predicted states: (2*ndim) * NV planes, state s = 2*axis + high
+    double* Fc[3];      // convective face flux per axis, NV planes each
+    double* Fv[3];      // viscous face flux per axis (3-D: slots 1..4; 2-D: slots 1..3)
+    long long nfa[3];   // faces per axis (plane stride of Fc / Fv)
+    DevState* st;
+};
+
+enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2 };
+enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
+enum { CTL_BEGIN_STEP = 0, CTL_STAGE1_DT = 1, CTL_STAGE_S = 2, CTL_END_RK = 3, CTL_SEED = 4, CTL_HANCOCK_DT = 5, CTL_END_HANCOCK = 6,
+       CTL_RESIDUAL_S = 7 };
+
+}   // namespace fvg

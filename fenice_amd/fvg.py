@@ -1,0 +1,235 @@
+"""Python binding of the device library (libfvg.so) through its C ABI (include/fvg.h).
+
+`Solver` mirrors the reference's step interface for the uniform path: `residual()` is
+Simulation::residual (cpp/src/Simulation.cpp:1351) returning (R, S_max), `step()` is TimeScheme::step
+(cpp/include/fv/TimeScheme.hpp:48) returning StepResult(dt, s_max), and failures raise the exception kinds
+the reference raises (ValueError for std::invalid_argument, LogicError for std::logic_error, Diverged for
+the runtime_error of check_progress). There is no CPU fallback: without the built library this module raises
+on import of the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .config import Config, EDIVERGED, EHIP, EINVAL, ELOGIC, ENOMEM, EUNSUPPORTED, OK
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfvg.so")
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+EXPORTS = [
+    "fvg_config_defaults", "fvg_config_set", "fvg_config_finalize", "fvg_dt_length", "fvg_create", "fvg_destroy",
+    "fvg_last_error", "fvg_upload", "fvg_download", "fvg_residual", "fvg_step", "fvg_advance", "fvg_get_lagged_speed",
+    "fvg_set_lagged_speed", "fvg_conserved_totals", "fvg_kernel_times", "fvg_set_timing", "fvg_riemann_batch",
+]
+
+
+class FvgError(RuntimeError):
+    code = -1
+
+
+class LogicError(FvgError):
+    code = ELOGIC
+
+
+class Diverged(FvgError):
+    code = EDIVERGED
+
+
+class HipError(FvgError):
+    code = EHIP
+
+
+class OutOfMemory(FvgError):
+    code = ENOMEM
+
+
+class Unsupported(FvgError):
+    code = EUNSUPPORTED
+
+
+class InvalidArgument(FvgError, ValueError):
+    code = EINVAL
+
+
+_ERR = {EINVAL: InvalidArgument, ELOGIC: LogicError, EDIVERGED: Diverged, EHIP: HipError, ENOMEM: OutOfMemory,
+        EUNSUPPORTED: Unsupported}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (built by fenice_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(Config)
+        L.fvg_config_defaults.argtypes = [P]
+        L.fvg_config_set.argtypes = [P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]
+        L.fvg_config_finalize.argtypes = [P, C.c_char_p, C.c_size_t]
+        L.fvg_dt_length.argtypes = [P]
+        L.fvg_dt_length.restype = C.c_double
+        L.fvg_create.argtypes = [P, C.c_int, C.POINTER(C.c_void_p)]
+        L.fvg_destroy.argtypes = [C.c_void_p]
+        L.fvg_last_error.argtypes = [C.c_void_p]
+        L.fvg_last_error.restype = C.c_char_p
+        L.fvg_upload.argtypes = [C.c_void_p, _D, C.c_size_t, C.c_int]
+        L.fvg_download.argtypes = [C.c_void_p, _D, C.c_size_t, C.c_int]
+        L.fvg_residual.argtypes = [C.c_void_p, C.c_double, _D, C.c_int, _D]
+        L.fvg_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _D, _D]
+        L.fvg_advance.argtypes = [C.c_void_p, C.c_int, _D, C.c_double, C.c_double, _I, _D]
+        L.fvg_get_lagged_speed.argtypes = [C.c_void_p, _D]
+        L.fvg_set_lagged_speed.argtypes = [C.c_void_p, C.c_double]
+        L.fvg_conserved_totals.argtypes = [C.c_void_p, _D]
+        L.fvg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), _D, C.POINTER(C.c_longlong), C.c_int]
+        L.fvg_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.fvg_riemann_batch.argtypes = [P, C.c_int, _D, _D, _I, C.c_size_t, _D, _D]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, ctx=None):
+    if rc == OK:
+        return
+    msg = lib().fvg_last_error(ctx).decode(errors="replace")
+    raise _ERR.get(rc, FvgError)(msg or f"status {rc}")
+
+
+def config_from_keys(keys: dict) -> Config:
+    """A Config from the reference's run keys and labels (fvg_config_set), with `auto` resolved."""
+    L = lib()
+    c = Config()
+    L.fvg_config_defaults(C.byref(c))
+    err = C.create_string_buffer(512)
+    for k, v in keys.items():
+        if isinstance(v, bool):
+            v = "true" if v else "false"
+        rc = L.fvg_config_set(C.byref(c), str(k).encode(), str(v).encode(), err, 512)
+        if rc != OK:
+            raise InvalidArgument(err.value.decode())
+    rc = L.fvg_config_finalize(C.byref(c), err, 512)
+    if rc != OK:
+        raise InvalidArgument(err.value.decode())
+    return c
+
+
+@dataclass
+class StepResult:
+    """TimeScheme.hpp:17-20"""
+    dt: float
+    s_max: float
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class Solver:
+    """One device context: the resident state of a uniform mesh and the configured residual and time scheme."""
+
+    def __init__(self, cfg: Config, device: int = 0):
+        self.cfg = cfg.copy()
+        h = C.c_void_p()
+        _check(lib().fvg_create(C.byref(self.cfg), device, C.byref(h)))
+        self._h = h
+        self.ncells = cfg.nx * cfg.ny * cfg.nz
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    @property
+    def dt_length(self) -> float:
+        return lib().fvg_dt_length(C.byref(self.cfg))
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.ndim != 2 or U.shape[0] != self.ncells:
+            raise InvalidArgument(f"U must be ({self.ncells}, 4|5)")
+        self._c(lib().fvg_upload(self._h, _dp(U), U.shape[0], U.shape[1]))
+
+    def download(self, ncomp: int = 5):
+        U = np.empty((self.ncells, ncomp))
+        self._c(lib().fvg_download(self._h, _dp(U), self.ncells, ncomp))
+        return U
+
+    def residual(self, predictor_half_dt: float = 0.0, ncomp: int = 5):
+        """(R, S_max) of the resident state (Simulation::residual)."""
+        R = np.empty((self.ncells, ncomp))
+        S = C.c_double()
+        self._c(lib().fvg_residual(self._h, predictor_half_dt, _dp(R), ncomp, C.byref(S)))
+        return R, S.value
+
+    def step(self, t: float, t_max: float, dt_max: float = 0.0) -> StepResult:
+        dt, S = C.c_double(), C.c_double()
+        self._c(lib().fvg_step(self._h, t, t_max, dt_max, C.byref(dt), C.byref(S)))
+        return StepResult(dt.value, S.value)
+
+    def advance(self, t: float, t_end: float, max_steps: int = 0, dt_max: float = 0.0):
+        """Steps while t < t_end (at most max_steps > 0); returns (t, steps, s_last)."""
+        tt = C.c_double(t)
+        n = C.c_int()
+        s = C.c_double()
+        self._c(lib().fvg_advance(self._h, max_steps, C.byref(tt), t_end, dt_max, C.byref(n), C.byref(s)))
+        return tt.value, n.value, s.value
+
+    @property
+    def lagged_speed(self) -> float:
+        s = C.c_double()
+        self._c(lib().fvg_get_lagged_speed(self._h, C.byref(s)))
+        return s.value
+
+    @lagged_speed.setter
+    def lagged_speed(self, s: float):
+        self._c(lib().fvg_set_lagged_speed(self._h, s))
+
+    def conserved_totals(self):
+        out = np.empty(3)
+        self._c(lib().fvg_conserved_totals(self._h, _dp(out)))
+        return out
+
+    def set_timing(self, on: bool):
+        self._c(lib().fvg_set_timing(self._h, int(on)))
+
+    def kernel_times(self):
+        names = (C.c_char_p * 16)()
+        ms = (C.c_double * 16)()
+        n = (C.c_longlong * 16)()
+        k = lib().fvg_kernel_times(self._h, names, ms, n, 16)
+        return {names[i].decode(): (ms[i], n[i]) for i in range(k)}
+
+
+def riemann_batch(cfg: Config, UL, UR, axis, device: int = 0):
+    """Device Riemann fluxes of independent faces (the solver of cfg): (F, smax)."""
+    UL = np.ascontiguousarray(UL, np.float64)
+    UR = np.ascontiguousarray(UR, np.float64)
+    ax = np.ascontiguousarray(axis, np.int32)
+    n = UL.shape[0]
+    F = np.empty((n, 5))
+    S = np.empty(n)
+    _check(lib().fvg_riemann_batch(C.byref(cfg), device, _dp(UL), _dp(UR), ax.ctypes.data_as(_I), n, _dp(F), _dp(S)))
+    return F, S

@@ -1,0 +1,111 @@
+"""Configurations shared by the parity tests: reference run keys (labels as in the case files) and the
+seeded states they are checked on."""
+from __future__ import annotations
+
+import glob
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+TWO_PI = "6.283185307179586"
+
+PERIODIC2 = dict(bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic")
+PERIODIC3 = dict(PERIODIC2, bc_zmin="periodic", bc_zmax="periodic")
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name):
+    d = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    return dict(U0=d["U0"], U_ref=d["U_ref"], step=int(d["step"]), keys=json.loads(str(d["keys"])), case=str(d["case"]))
+
+
+def smooth_state(nx, ny, nz, seed, gamma=1.4, noise=0.02, mach=0.3):
+    """A smooth, non-trivial conservative state (n, 5) with seeded noise; rho*w = +0 in 2-D."""
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x = (i.ravel() + 0.5) / nx * 2 * np.pi
+    y = (j.ravel() + 0.5) / ny * 2 * np.pi
+    z = (k.ravel() + 0.5) / max(nz, 1) * 2 * np.pi
+    n = nx * ny * nz
+    rho = 1.0 + 0.2 * np.sin(x) * np.cos(y) + noise * rng.standard_normal(n)
+    u = mach * (np.sin(y + 0.3) + noise * rng.standard_normal(n))
+    v = mach * (np.cos(x) * np.sin(z) + noise * rng.standard_normal(n))
+    w = mach * (np.sin(x + y) + noise * rng.standard_normal(n)) if nz > 1 else np.zeros(n)
+    p = 1.0 + 0.1 * np.cos(x + 2 * y) + noise * np.abs(rng.standard_normal(n))
+    U = np.empty((n, 5))
+    U[:, 0] = rho
+    U[:, 1] = rho * u
+    U[:, 2] = rho * v
+    U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    U[:, 4] = rho * w if nz > 1 else 0.0
+    return U
+
+
+def shock_state(nx, ny, nz, seed, gamma=1.4):
+    """A state with strong jumps (pressure ratio 10, density ratio 8) plus seeded noise: exercises the shock blend,
+    the limiters' clipping and the Hancock admissibility reverts."""
+    U = smooth_state(nx, ny, nz, seed, gamma, noise=0.01, mach=0.5)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    left = (i.ravel() + j.ravel()) < (nx + ny) // 2
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    U[:, 0] *= rho
+    U[:, 1] *= rho
+    U[:, 2] *= rho
+    U[:, 4] *= rho
+    kin = 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 4] ** 2) / U[:, 0]
+    U[:, 3] = p / (gamma - 1.0) + kin
+    return U
+
+
+def base_keys(nx, ny, nz=1, periodic=True, **kw):
+    k = dict(nx=nx, ny=ny, nz=nz, Lx=TWO_PI, Ly=TWO_PI, Lz=TWO_PI, gamma=1.4, use_viscous="false", riemann_solver="HLLC",
+             reconstruction="muscl", muscl_limiter="Van Leer", time_scheme="SSP Runge-Kutta 3", cfl=0.5, boundary_order=1)
+    if periodic:
+        k.update(PERIODIC3 if nz > 1 else PERIODIC2)
+    k.update(kw)
+    return {a: str(b) for a, b in k.items()}
+
+
+# (id, keys, state kind): the configuration matrix of the parity tests
+def matrix():
+    out = []
+    for lim in ("Minmod", "Van Leer", "MC", "Koren"):
+        out.append((f"2d-muscl-{lim}", base_keys(12, 10, muscl_limiter=lim), "shock"))
+    for rc in ("weno3", "weno5", "none"):
+        out.append((f"2d-{rc}", base_keys(12, 10, reconstruction=rc, time_scheme="SSP Runge-Kutta 3" if rc != "none" else "Explicit Euler"), "shock"))
+    out.append(("2d-hancock", base_keys(14, 12, time_scheme="MUSCL-Hancock"), "shock"))
+    out.append(("2d-lowmach-rk2", base_keys(12, 10, riemann_solver="HLLC (Low-Mach Thornber)", time_scheme="SSP Runge-Kutta 2"), "smooth"))
+    out.append(("2d-visc-periodic", base_keys(12, 10, use_viscous="true", mu="0.02", time_scheme="SSP Runge-Kutta 2"), "smooth"))
+    walls = dict(bc_xmin="transmissive", bc_xmax="wall", bc_ymin="no_slip", bc_ymax="wall")
+    for bo in (1, 2):
+        out.append((f"2d-walls-muscl-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, **walls), "shock"))
+        out.append((f"2d-walls-weno5-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, reconstruction="weno5", **walls), "shock"))
+        out.append((f"2d-walls-hancock-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, time_scheme="MUSCL-Hancock", **walls), "shock"))
+        out.append((f"2d-walls-visc-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, use_viscous="true", mu="0.01",
+                                                       **dict(walls, bc_ymax="moving_wall", u_in="1.0")), "smooth"))
+    out.append(("2d-first-order-walls", base_keys(12, 10, periodic=False, reconstruction="none", time_scheme="Explicit Euler", **walls), "shock"))
+    for rc in ("weno5", "weno3", "muscl"):
+        out.append((f"3d-{rc}-visc-lm", base_keys(8, 9, 7, reconstruction=rc, use_viscous="true", mu="0.01", pr="0.71",
+                                                    riemann_solver="HLLC (Low-Mach Thornber)"), "smooth"))
+    out.append(("3d-weno5-hllc-shock", base_keys(8, 9, 7, reconstruction="weno5"), "shock"))
+    out.append(("3d-hancock", base_keys(8, 9, 7, time_scheme="MUSCL-Hancock"), "shock"))
+    out.append(("3d-hancock-visc", base_keys(8, 9, 7, time_scheme="MUSCL-Hancock", use_viscous="true", mu="0.01"), "smooth"))
+    walls3 = dict(bc_xmin="transmissive", bc_xmax="wall", bc_ymin="no_slip", bc_ymax="wall", bc_zmin="wall", bc_zmax="no_slip")
+    for bo in (1, 2):
+        out.append((f"3d-walls-weno5-visc-bo{bo}", base_keys(8, 9, 7, periodic=False, boundary_order=bo, reconstruction="weno5",
+                                                             use_viscous="true", mu="0.01", **walls3), "smooth"))
+        out.append((f"3d-walls-hancock-bo{bo}", base_keys(8, 9, 7, periodic=False, boundary_order=bo, time_scheme="MUSCL-Hancock",
+                                                          **walls3), "shock"))
+    return out
+
+
+def state_for(kind, keys, seed=7):
+    nx, ny, nz = int(keys["nx"]), int(keys["ny"]), int(keys.get("nz", 1))
+    return (shock_state if kind == "shock" else smooth_state)(nx, ny, nz, seed)

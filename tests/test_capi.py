@@ -1,0 +1,91 @@
+"""The C ABI without a device: the library loads, exports every entry point include/fvg.h declares, and parses
+the reference's keys, labels and aliases with the reference's error behaviour."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from fenice_amd import fvg
+from fenice_amd.config import (BC_MOVING_WALL, BC_NO_SLIP, BC_PERIODIC, BC_TRANSMISSIVE, BC_WALL, HLLC, LIM_KOREN, LIM_MC,
+                               LIM_MINMOD, LIM_VAN_LEER, LOWMACH_HLLC, RECON_MUSCL, RECON_NONE, RECON_WENO3, RECON_WENO5, TS_EULER,
+                               TS_HANCOCK, TS_SSPRK2, TS_SSPRK3, Config)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "fvg.h")).read()
+    return sorted(set(re.findall(r"^(?:int|void|double|const char\s*\*)\s*(fvg_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    L = fvg.lib()
+    names = declared_entry_points()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(fvg.EXPORTS) <= set(names)
+
+
+def test_config_struct_size_matches_header():
+    # the ctypes mirror and the C struct agree: defaults written by C read back field by field
+    c = Config()
+    fvg.lib().fvg_config_defaults(C.byref(c))
+    d = Config.defaults()
+    for f, _ in Config._fields_:
+        if f == "bc":
+            assert list(c.bc) == list(d.bc)
+        elif f != "time_scheme":
+            assert getattr(c, f) == getattr(d, f), f
+
+
+@pytest.mark.parametrize("label,expect", [("HLLC", HLLC), ("hllc", HLLC), ("HLLC (Low-Mach Thornber)", LOWMACH_HLLC),
+                                          ("lowmach", LOWMACH_HLLC), ("Low-Mach", LOWMACH_HLLC)])
+def test_riemann_labels(label, expect):
+    assert fvg.config_from_keys(dict(nx=4, ny=4, riemann_solver=label)).riemann == expect
+
+
+def test_labels_and_auto_scheme():
+    c = fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="muscl", muscl_limiter="Van Leer"))
+    assert c.reconstruction == RECON_MUSCL and c.limiter == LIM_VAN_LEER and c.time_scheme == TS_HANCOCK
+    c = fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="MUSCL", use_viscous="true"))
+    assert c.time_scheme == TS_SSPRK2
+    c = fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno5"))
+    assert c.reconstruction == RECON_WENO5 and c.time_scheme == TS_EULER
+    c = fvg.config_from_keys(dict(nx=4, ny=4))
+    assert c.reconstruction == RECON_NONE and c.time_scheme == TS_EULER
+    for lab, lim in [("Minmod", LIM_MINMOD), ("vanleer", LIM_VAN_LEER), ("MC", LIM_MC), ("koren", LIM_KOREN)]:
+        assert fvg.config_from_keys(dict(nx=4, ny=4, muscl_limiter=lab)).limiter == lim
+    for lab, ts in [("SSP Runge-Kutta 3", TS_SSPRK3), ("rk2", TS_SSPRK2), ("Explicit Euler", TS_EULER), ("hancock", TS_HANCOCK)]:
+        assert fvg.config_from_keys(dict(nx=4, ny=4, time_scheme=lab)).time_scheme == ts
+    assert fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno3")).reconstruction == RECON_WENO3
+    c = fvg.config_from_keys(dict(nx=4, ny=4, bc_xmin="Inviscid Wall (Slip)", bc_xmax="no_slip", bc_ymin="Moving Wall",
+                                  bc_ymax="transmissive"))
+    assert list(c.bc)[:4] == [BC_WALL, BC_NO_SLIP, BC_MOVING_WALL, BC_TRANSMISSIVE]
+    c = fvg.config_from_keys(dict(nx=4, ny=4, nz=4, bc_zmin="periodic", bc_zmax="periodic"))
+    assert c.bc[4] == BC_PERIODIC and c.is3d
+
+
+@pytest.mark.parametrize("keys,msg", [
+    (dict(nx=4, ny=4, riemann_solver="roe"), "Unknown Riemann solver"),
+    (dict(nx=4, ny=4, riemann_solver="HLLC (Preconditioned)"), "removed"),
+    (dict(nx=4, ny=4, muscl_limiter="superbee"), "Unknown MUSCL limiter"),
+    (dict(nx=4, ny=4, reconstruction="weno7"), "Unknown reconstruction"),
+    (dict(nx=4, ny=4, time_scheme="rk4"), "Unknown time scheme"),
+    (dict(nx=4, ny=4, bc_xmin="sticky"), "Unknown boundary condition"),
+    (dict(nx=4, ny=4, cfl="fast"), "not a number"),
+    (dict(nx=4, ny=4, no_such_key=1), "Unknown key"),
+    (dict(nx=0, ny=4), "positive"),
+    (dict(nx=4, ny=4, bc_xmin="periodic", bc_xmax="wall"), "pairs"),
+])
+def test_invalid_keys_raise_value_error(keys, msg):
+    with pytest.raises(ValueError, match=msg):
+        fvg.config_from_keys(keys)
+
+
+def test_dt_length():
+    c = fvg.config_from_keys(dict(nx=8, ny=4, Lx=1.0, Ly=1.0))
+    assert fvg.lib().fvg_dt_length(C.byref(c)) == 0.125
+    c = fvg.config_from_keys(dict(nx=4, ny=4, nz=4, Lx=1.0, Ly=1.0, Lz=1.0))
+    assert fvg.lib().fvg_dt_length(C.byref(c)) == 2.0 / (4.0 + 4.0 + 4.0)

@@ -1,0 +1,167 @@
+"""The device path against the oracle, bit for bit, and against the reference's stored states.
+
+FP64 parity bar: identical bits (the device code keeps the reference's operation order with no fused
+multiply-add and IEEE division / square root). 2-D fields carry no rho*w plane on the device; its slot is +0 on
+download and is compared by value."""
+import sys
+
+import numpy as np
+import pytest
+
+from cases import golden_names, load_golden, matrix, state_for, base_keys, smooth_state
+from fenice_amd import fvg
+from fenice_amd.config import Config, EOS_STIFFENED, HLLC, LOWMACH_HLLC
+from riemann_pool import HASHES, fingerprint, pool
+
+pytestmark = pytest.mark.gpu
+MATRIX = matrix()
+
+
+def same_bits(a, b, two_d=False):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    if two_d:
+        return np.array_equal(a[:, :4].view(np.uint64), b[:, :4].view(np.uint64)) and np.array_equal(a[:, 4], b[:, 4])
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("kind", ["ideal", "stiffened"])
+@pytest.mark.parametrize("solver", ["hllc", "lowmach"])
+def test_device_riemann_fingerprint(kind, solver):
+    say(f"riemann fingerprint {kind} {solver}")
+    UL, UR, ax, gamma, p_inf, stiff = pool(kind)
+    cfg = Config.make(gamma=gamma, p_inf=p_inf, eos_type=EOS_STIFFENED if stiff else 0, shock_blend=0.0,
+                      riemann=LOWMACH_HLLC if solver == "lowmach" else HLLC)
+    F, S = fvg.riemann_batch(cfg, UL, UR, ax)
+    assert fingerprint(F, S) == HASHES[(kind, solver)]
+
+
+@pytest.mark.parametrize("name,keys,kind", MATRIX, ids=[m[0] for m in MATRIX])
+def test_residual_and_steps_match_oracle(oracle, name, keys, kind):
+    say(f"parity {name}")
+    cfg = fvg.config_from_keys(keys)
+    two_d = cfg.nz == 1
+    U = state_for(kind, keys)
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        Rd, Sd = s.residual()
+        Ro, So = oracle.residual(cfg, U)
+        assert Sd == So
+        assert same_bits(Rd, Ro, two_d)
+        if keys["time_scheme"] == "MUSCL-Hancock":
+            Rd, Sd = s.residual(0.002)
+            Ro, So = oracle.residual(cfg, U, 0.002)
+            assert Sd == So and same_bits(Rd, Ro, two_d)
+        Uo = U.copy()
+        t, lag = 0.0, 0.0
+        for _ in range(3):
+            r = s.step(t, 10.0)
+            dto, so, lag, ok = oracle.step(cfg, Uo, t, 10.0, 0.0, lag)
+            assert ok and r.dt == dto and r.s_max == so
+            t += r.dt
+        assert same_bits(s.download(), Uo, two_d)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_device_reproduces_reference_state(oracle, name):
+    say(f"golden {name}")
+    g = load_golden(name)
+    cfg = fvg.config_from_keys(g["keys"])
+    t_end = float(g["keys"]["t_end"])
+    with fvg.Solver(cfg) as s:
+        s.upload(g["U0"])
+        t, steps, _ = s.advance(0.0, t_end)
+        U = s.download()
+    assert steps == g["step"]
+    Uref = g["U_ref"]
+    scale = np.maximum(np.max(np.abs(Uref), axis=0), 1e-30)
+    assert float(np.max(np.abs(U[:, : Uref.shape[1]] - Uref) / scale)) < 1e-8
+    Uo = g["U0"].copy()
+    oracle.run(cfg, Uo, t_end)
+    assert same_bits(U, Uo, cfg.nz == 1)
+
+
+def test_advance_equals_single_steps():
+    say("advance vs step")
+    keys = base_keys(16, 12, 10, reconstruction="weno5", use_viscous="true", mu="0.01")
+    cfg = fvg.config_from_keys(keys)
+    U = smooth_state(16, 12, 10, 4)
+    with fvg.Solver(cfg) as a, fvg.Solver(cfg) as b:
+        a.upload(U)
+        b.upload(U)
+        t, n, _ = a.advance(0.0, 1.0, max_steps=5)
+        tb = 0.0
+        for _ in range(5):
+            tb += b.step(tb, 1.0).dt
+        assert n == 5 and t == tb
+        assert same_bits(a.download(), b.download())
+
+
+def test_totals_match_volume_averages(oracle):
+    say("totals")
+    keys = base_keys(20, 18, 16, reconstruction="weno5")
+    cfg = fvg.config_from_keys(keys)
+    U = smooth_state(20, 18, 16, 11)
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        tot = s.conserved_totals()
+    ref = oracle.volume_averages(cfg, U)
+    assert np.allclose(tot, ref, rtol=1e-13, atol=0)
+
+
+def test_divergence_is_reported(oracle):
+    # a cell with infinite energy gives an infinite wave speed, a zero time step and the reference's
+    # check_progress error (a NaN face alone would not: the reference's maximum drops NaN); forced from the host,
+    # every kernel still exits normally
+    say("divergence")
+    keys = base_keys(12, 10, reconstruction="muscl")
+    cfg = fvg.config_from_keys(keys)
+    U = smooth_state(12, 10, 1, 2)
+    U[17] = [1.0, 0.0, 0.0, np.inf, 0.0]
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        with pytest.raises(fvg.Diverged):
+            s.step(0.0, 1.0)
+    _, _, _, ok = oracle.step(cfg, U.copy(), 0.0, 1.0)
+    assert not ok
+
+
+def test_bench_shape_step_matches_oracle(oracle):
+    # the benchmark's scheme (3-D Taylor-Green, WENO5-Z + HLLC + SSP-RK3, Navier-Stokes, Pr 0.71) at 48^3, two steps
+    say("tgv 48^3 parity")
+    n = 48
+    keys = base_keys(n, n, n, reconstruction="weno5", use_viscous="true", mu="6.25e-4", pr="0.71", cfl="0.5")
+    cfg = fvg.config_from_keys(keys)
+    U = smooth_state(n, n, n, 21, mach=0.1)
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        t, steps, _ = s.advance(0.0, 100.0, max_steps=2)
+        Ud = s.download()
+    Uo = U.copy()
+    oracle.run(cfg, Uo, 100.0, max_steps=2)
+    assert same_bits(Ud, Uo)
+
+
+def test_large_run_is_deterministic_and_conservative():
+    # 128^3 periodic: two runs give the same bits (no race in reductions), mass and energy are conserved to round-off
+    say("128^3 determinism")
+    n = 128
+    keys = base_keys(n, n, n, reconstruction="weno5", use_viscous="true", mu="6.25e-4", pr="0.71")
+    cfg = fvg.config_from_keys(keys)
+    U = smooth_state(n, n, n, 5, mach=0.2)
+    out = []
+    for _ in range(2):
+        with fvg.Solver(cfg) as s:
+            s.upload(U)
+            tot0 = s.conserved_totals()
+            s.advance(0.0, 100.0, max_steps=3)
+            out.append((s.download(), s.conserved_totals()))
+    assert same_bits(out[0][0], out[1][0])
+    tot1 = out[0][1]
+    assert abs(tot1[0] - tot0[0]) <= 1e-13 * abs(tot0[0])
+    assert abs(tot1[2] - tot0[2]) <= 1e-13 * abs(tot0[2])
