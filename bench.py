@@ -122,6 +122,21 @@ def cpu_baseline(n: int, budget_s: float):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def copy_calibration(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of known size (HIP's copy kernel), so that a counter run can scale FETCH_SIZE /
+    WRITE_SIZE: the copy moves nbytes in and nbytes out."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    hip.hipMalloc(C.byref(a), C.c_size_t(nbytes))
+    hip.hipMalloc(C.byref(b), C.c_size_t(nbytes))
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +147,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="end with a 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
     if a.cpu_worker:
         print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget)))
@@ -183,6 +199,8 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_budget)
     print(json.dumps(out), flush=True)
+    if a.calibrate_copy:
+        copy_calibration()
 
 
 if __name__ == "__main__":

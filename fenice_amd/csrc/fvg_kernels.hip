@@ -39,6 +39,16 @@ template <int NV> __device__ __forceinline__ void load_prim(const KP& p, long lo
     for (int q = 0; q < NV; ++q) w[q] = p.W[q * p.nc + c];
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs of the chip, each with its own L2. The logical block of a
+// dispatched block gives every XCD a contiguous range of logical blocks (rows and planes of the mesh), so that the
+// stencil rows a block shares with the next one are found in that XCD's L2. The tail that does not divide by 8
+// keeps its order.
+__device__ __forceinline__ long long xcd_block() {
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb >> 3;
+    if (b >= per * 8u) return b;
+    return (long long)(b & 7u) * per + (b >> 3);
+}
+
 // Wave-level then block-level maximum of non-negative, non-NaN doubles, then one atomicMax of the bit pattern.
 __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
     __shared__ double red[16];
@@ -205,7 +215,7 @@ __device__ __forceinline__ void visc_pad_prim(const KP& p, int i, int j, int k, 
 // ------------------------------------------------------------------------------------------------------------
 template <int NV>
 __global__ __launch_bounds__(256) void k_cells(KP p) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long c = xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
     if (c < p.nc && !p.st->done) {
         double u[NV], w[NV];
@@ -269,7 +279,7 @@ __device__ __forceinline__ void euler_flux_axis(const Eos& e, const double* U, i
 template <int NV>
 __global__ __launch_bounds__(256) void k_hancock(KP p) {
     constexpr int ND = NV - 2, NS = 2 * ND;
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long c = xcd_block() * blockDim.x + threadIdx.x;
     unsigned long long reverted = 0;
     if (c < p.nc && !p.st->done) {
         const int i = (int)(c % p.nx);
@@ -477,7 +487,7 @@ template <int NV, int D, int MODE>
 __global__ __launch_bounds__(256) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
-    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long f = xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
     if (f < nf && !p.st->done) {
         const int i = (int)(f % ni);
@@ -574,7 +584,7 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
 // ------------------------------------------------------------------------------------------------------------
 template <int NV, int UPD>
 __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= p.nc || p.st->done) return;
     const int i = (int)(c % p.nx);
     const long long r = c / p.nx;
