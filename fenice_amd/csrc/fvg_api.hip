@@ -12,13 +12,13 @@
 #include <vector>
 
 #include "../../include/fvg.h"
-#include "fvg_kernels.hpp"
+#include "fvg_host.hpp"
 
 namespace fvg {
-template <int NV> __global__ void k_cells(KP);
+template <int NV, typename T> __global__ void k_cells(KP);
 template <int NV> __global__ void k_hancock(KP);
-template <int NV, int D, int MODE> __global__ void k_faces(KP);
-template <int NV, int UPD> __global__ void k_update(KP, const double*, const double*, double*);
+template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
+template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV> __global__ void k_totals_partial(const double*, long long, double*);
 __global__ void k_totals_final(const double*, int, long long, double*);
 __global__ void k_control(KP, int, int);
@@ -49,8 +49,10 @@ struct fvg_ctx {
     hipStream_t stream = nullptr;
     int NV = 5, ND = 3;
     long long nc = 0, nfa[3] = {0, 0, 0};
-    double *U = nullptr, *Ut = nullptr, *Rb = nullptr, *W = nullptr, *G = nullptr, *HS = nullptr;
-    double *Fc[3] = {nullptr, nullptr, nullptr}, *Fv[3] = {nullptr, nullptr, nullptr};
+    double *U = nullptr, *Ut = nullptr, *Rb = nullptr, *HS = nullptr;
+    void *W = nullptr, *G = nullptr;                                          // double, or float with fp32 fluxes
+    void *Fc[3] = {nullptr, nullptr, nullptr}, *Fv[3] = {nullptr, nullptr, nullptr};
+    bool fp32 = false;
     double *tot_part = nullptr, *tot_out = nullptr;
     DevState* st = nullptr;
     DevState* st_host = nullptr;   // pinned mirror for the read-back
@@ -134,19 +136,18 @@ KP make_kp(fvg_ctx* c, const double* Uin) {
     return p;
 }
 
-template <int NV>
+template <int NV, typename T>
 int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode) {
     const KP p = make_kp(c, Uin);
     int rc;
-    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL(k_cells<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
         return rc;
     if (mode == MODE_HANCOCK)
         if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
             return rc;
 #define FACES(D, M)                                                                                                                       \
-    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, M>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
-#define AXIS(D)                                                                             \
-    (mode == MODE_PLAIN ? FACES(D, MODE_PLAIN) : (mode == MODE_HANCOCK ? FACES(D, MODE_HANCOCK) : FACES(D, MODE_FIRST_ORDER)))
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, M, T>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+#define AXIS(D) (mode == MODE_PLAIN ? FACES(D, MODE_PLAIN) : FACES(D, MODE_FIRST_ORDER))
     if ((rc = AXIS(0))) return rc;
     if ((rc = AXIS(1))) return rc;
     if constexpr (NV == 5)
@@ -156,8 +157,37 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode) {
     return FVG_OK;
 }
 
+// MUSCL-Hancock faces (FP64 only)
+template <int NV>
+int enqueue_hancock_faces(fvg_ctx* c, const KP& p) {
+    int rc;
+#define HFACES(D)                                                                                                                         \
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, MODE_HANCOCK, double>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, \
+                                                        c->stream, p); })
+    if ((rc = HFACES(0))) return rc;
+    if ((rc = HFACES(1))) return rc;
+    if constexpr (NV == 5)
+        if ((rc = HFACES(2))) return rc;
+#undef HFACES
+    return FVG_OK;
+}
+
+template <int NV>
+int enqueue_hancock_residual(fvg_ctx* c, const double* Uin) {
+    const KP p = make_kp(c, Uin);
+    int rc;
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, double>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+        return rc;
+    if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+        return rc;
+    return enqueue_hancock_faces<NV>(c, p);
+}
+
 int enqueue_residual(fvg_ctx* c, const double* Uin, int mode) {
-    return c->NV == 5 ? enqueue_residual_nv<5>(c, Uin, mode) : enqueue_residual_nv<4>(c, Uin, mode);
+    if (mode == MODE_HANCOCK)
+        return c->NV == 5 ? enqueue_hancock_residual<5>(c, Uin) : enqueue_hancock_residual<4>(c, Uin);
+    if (c->fp32) return c->NV == 5 ? enqueue_residual_nv<5, float>(c, Uin, mode) : enqueue_residual_nv<4, float>(c, Uin, mode);
+    return c->NV == 5 ? enqueue_residual_nv<5, double>(c, Uin, mode) : enqueue_residual_nv<4, double>(c, Uin, mode);
 }
 
 int enqueue_control(fvg_ctx* c, int op, int arg) {
@@ -165,22 +195,23 @@ int enqueue_control(fvg_ctx* c, int op, int arg) {
     return launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, c->stream, p, op, arg); });
 }
 
-template <int NV>
+template <int NV, typename T>
 int enqueue_update_nv(fvg_ctx* c, int upd, const double* U0, const double* Ut, double* out) {
     const KP p = c->base;
     const dim3 g(grid_for(c->nc)), b(kBlock);
     return launch(c, FAM_UPDATE, [&] {
         switch (upd) {
-        case UPD_RESIDUAL: hipLaunchKernelGGL((k_update<NV, UPD_RESIDUAL>), g, b, 0, c->stream, p, U0, Ut, out); break;
-        case UPD_AXPY: hipLaunchKernelGGL((k_update<NV, UPD_AXPY>), g, b, 0, c->stream, p, U0, Ut, out); break;
-        case UPD_RK2_2: hipLaunchKernelGGL((k_update<NV, UPD_RK2_2>), g, b, 0, c->stream, p, U0, Ut, out); break;
-        case UPD_RK3_2: hipLaunchKernelGGL((k_update<NV, UPD_RK3_2>), g, b, 0, c->stream, p, U0, Ut, out); break;
-        default: hipLaunchKernelGGL((k_update<NV, UPD_RK3_3>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_RESIDUAL: hipLaunchKernelGGL((k_update<NV, UPD_RESIDUAL, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_AXPY: hipLaunchKernelGGL((k_update<NV, UPD_AXPY, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_RK2_2: hipLaunchKernelGGL((k_update<NV, UPD_RK2_2, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        case UPD_RK3_2: hipLaunchKernelGGL((k_update<NV, UPD_RK3_2, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
+        default: hipLaunchKernelGGL((k_update<NV, UPD_RK3_3, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
         }
     });
 }
 int enqueue_update(fvg_ctx* c, int upd, const double* U0, const double* Ut, double* out) {
-    return c->NV == 5 ? enqueue_update_nv<5>(c, upd, U0, Ut, out) : enqueue_update_nv<4>(c, upd, U0, Ut, out);
+    if (c->fp32) return c->NV == 5 ? enqueue_update_nv<5, float>(c, upd, U0, Ut, out) : enqueue_update_nv<4, float>(c, upd, U0, Ut, out);
+    return c->NV == 5 ? enqueue_update_nv<5, double>(c, upd, U0, Ut, out) : enqueue_update_nv<4, double>(c, upd, U0, Ut, out);
 }
 
 int plain_mode(const fvg_ctx* c) { return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN; }
@@ -250,8 +281,8 @@ void free_ctx(fvg_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->U, c->Ut, c->Rb, c->W, c->G, c->HS, c->Fc[0], c->Fc[1], c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out};
-    for (double* b : bufs)
+    void* bufs[] = {c->U, c->Ut, c->Rb, c->W, c->G, c->HS, c->Fc[0], c->Fc[1], c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out};
+    for (void* b : bufs)
         if (b) hipFree(b);
     if (c->st) hipFree(c->st);
     if (c->st_host) hipHostFree(c->st_host);
@@ -264,93 +295,30 @@ void free_ctx(fvg_ctx* c) {
     delete c;
 }
 
-int checked_alloc(fvg_ctx* c, double** p, size_t n) {
+template <typename P>
+int checked_alloc(fvg_ctx* c, P** p, size_t n, size_t elem = sizeof(double)) {
     if (n == 0) return FVG_OK;
-    hipError_t e = hipMalloc(p, n * sizeof(double));
+    hipError_t e = hipMalloc((void**)p, n * elem);
     if (e != hipSuccess) {
         *p = nullptr;
-        return set_err(c, FVG_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(double)) + " bytes: " + hipGetErrorString(e));
+        return set_err(c, FVG_ENOMEM, "hipMalloc of " + std::to_string(n * elem) + " bytes: " + hipGetErrorString(e));
     }
     return FVG_OK;
 }
 
 }   // namespace
 
-extern "C" {
+namespace fvg {
 
-const char* fvg_last_error(const fvg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
-
-int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
-    if (!cfg_in || !out) return set_err(nullptr, FVG_EINVAL, "null argument");
-    *out = nullptr;
-    fvg_config cfg = *cfg_in;
-    char msg[256] = {0};
-    if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return set_err(nullptr, FVG_EINVAL, msg);
-    for (int s = 0; s < 6; ++s) {
-        const int b = cfg.bc[s];
-        if ((s < 4 || cfg.nz > 1) && (b == FVG_BC_NONREFLECTING || b == FVG_BC_MASSFLOW_INLET))
-            return set_err(nullptr, FVG_EUNSUPPORTED, "boundary type of side " + std::to_string(s) +
-                                                          " (non-reflecting / mass-flow inlet) is not on the device path");
-    }
-    if (cfg.time_scheme == FVG_TS_HANCOCK && (cfg.reconstruction == FVG_RECON_WENO3 || cfg.reconstruction == FVG_RECON_WENO5))
-        return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
-    if (cfg.flux_precision != FVG_FLUX_FP64)
-        return set_err(nullptr, FVG_EUNSUPPORTED, "flux_precision = fp32 is not on the device path yet");
-    const double faces = (double)(cfg.nx + 1) * cfg.ny * cfg.nz;
-    if (faces * 64 > 9.0e18) return set_err(nullptr, FVG_EINVAL, "mesh too large");
-
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(nullptr, FVG_EHIP, "no HIP device");
-    if (device < 0 || device >= ndev) return set_err(nullptr, FVG_EINVAL, "device index out of range");
-    fvg_ctx* c = new fvg_ctx();
-    c->cfg = cfg;
-    c->device = device;
-    if (hipSetDevice(device) != hipSuccess) {
-        free_ctx(c);
-        return set_err(nullptr, FVG_EHIP, "hipSetDevice failed");
-    }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        free_ctx(c);
-        return set_err(nullptr, FVG_EHIP, "hipStreamCreate failed");
-    }
+// Mesh, numerics and viscous scalars of the kernels for a finalized configuration (buffers are set by the caller).
+void fill_kernel_params(const fvg_config& cfg, KP& p) {
     const bool three = cfg.nz > 1;
-    c->NV = three ? 5 : 4;
-    c->ND = three ? 3 : 2;
-    c->nc = (long long)cfg.nx * cfg.ny * cfg.nz;
-    c->nfa[0] = (long long)(cfg.nx + 1) * cfg.ny * cfg.nz;
-    c->nfa[1] = (long long)cfg.nx * (cfg.ny + 1) * cfg.nz;
-    c->nfa[2] = three ? (long long)cfg.nx * cfg.ny * (cfg.nz + 1) : 0;
-    const int NV = c->NV, ND = c->ND;
-    const bool visc = cfg.use_viscous != 0, hancock = cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE;
-    int rc = FVG_OK;
-    rc = rc ? rc : checked_alloc(c, &c->U, (size_t)NV * c->nc);
-    rc = rc ? rc : checked_alloc(c, &c->Ut, (size_t)NV * c->nc);
-    rc = rc ? rc : checked_alloc(c, &c->Rb, (size_t)NV * c->nc);
-    rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * c->nc);
-    if (visc) rc = rc ? rc : checked_alloc(c, &c->G, (size_t)ND * ND * c->nc);
-    if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
-    for (int d = 0; d < ND; ++d) {
-        rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d]);
-        if (visc) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d]);
-    }
-    rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
-    rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
-    if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
-    if (!rc && hipHostMalloc(&c->st_host, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipHostMalloc of the state mirror");
-    if (rc) {
-        const std::string m = c->err;
-        free_ctx(c);
-        return set_err(nullptr, rc, m);
-    }
-    std::memset(c->st_host, 0, sizeof(DevState));
-
-    // kernel parameters that do not change during the run
-    KP& p = c->base;
+    const bool visc = cfg.use_viscous != 0;
     p.nx = cfg.nx;
     p.ny = cfg.ny;
     p.nz = cfg.nz;
     p.is3d = three;
-    p.nc = c->nc;
+    p.nc = (long long)cfg.nx * cfg.ny * cfg.nz;
     p.dx = cfg.Lx / cfg.nx;
     p.dy = cfg.Ly / cfg.ny;
     p.dz = cfg.Lz / cfg.nz;
@@ -405,6 +373,83 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     p.k_cond = cfg.mu * cfg.gamma / (cfg.pr * (cfg.gamma - 1.0));   // EOS::TransportLaw (Core.hpp:196)
     p.inv_h2 = three ? 1.0 / (p.dx * p.dx) + 1.0 / (p.dy * p.dy) + 1.0 / (p.dz * p.dz) : 1.0 / (p.dx * p.dx) + 1.0 / (p.dy * p.dy);
     for (int a = 0; a < 3; ++a) p.vper[a] = cfg.bc[2 * a] == FVG_BC_PERIODIC && cfg.bc[2 * a + 1] == FVG_BC_PERIODIC;
+}
+
+}   // namespace fvg
+
+extern "C" {
+
+const char* fvg_last_error(const fvg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
+    if (!cfg_in || !out) return set_err(nullptr, FVG_EINVAL, "null argument");
+    *out = nullptr;
+    fvg_config cfg = *cfg_in;
+    char msg[256] = {0};
+    if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return set_err(nullptr, FVG_EINVAL, msg);
+    for (int s = 0; s < 6; ++s) {
+        const int b = cfg.bc[s];
+        if ((s < 4 || cfg.nz > 1) && (b == FVG_BC_NONREFLECTING || b == FVG_BC_MASSFLOW_INLET))
+            return set_err(nullptr, FVG_EUNSUPPORTED, "boundary type of side " + std::to_string(s) +
+                                                          " (non-reflecting / mass-flow inlet) is not on the device path");
+    }
+    if (cfg.time_scheme == FVG_TS_HANCOCK && (cfg.reconstruction == FVG_RECON_WENO3 || cfg.reconstruction == FVG_RECON_WENO5))
+        return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
+    if (cfg.flux_precision == FVG_FLUX_FP32 && cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE)
+        return set_err(nullptr, FVG_EUNSUPPORTED, "flux_precision = fp32 with MUSCL-Hancock is not on the device path");
+    const double faces = (double)(cfg.nx + 1) * cfg.ny * cfg.nz;
+    if (faces * 64 > 9.0e18) return set_err(nullptr, FVG_EINVAL, "mesh too large");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(nullptr, FVG_EHIP, "no HIP device");
+    if (device < 0 || device >= ndev) return set_err(nullptr, FVG_EINVAL, "device index out of range");
+    fvg_ctx* c = new fvg_ctx();
+    c->cfg = cfg;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        free_ctx(c);
+        return set_err(nullptr, FVG_EHIP, "hipSetDevice failed");
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return set_err(nullptr, FVG_EHIP, "hipStreamCreate failed");
+    }
+    const bool three = cfg.nz > 1;
+    c->NV = three ? 5 : 4;
+    c->ND = three ? 3 : 2;
+    c->nc = (long long)cfg.nx * cfg.ny * cfg.nz;
+    c->nfa[0] = (long long)(cfg.nx + 1) * cfg.ny * cfg.nz;
+    c->nfa[1] = (long long)cfg.nx * (cfg.ny + 1) * cfg.nz;
+    c->nfa[2] = three ? (long long)cfg.nx * cfg.ny * (cfg.nz + 1) : 0;
+    const int NV = c->NV, ND = c->ND;
+    const bool visc = cfg.use_viscous != 0, hancock = cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE;
+    int rc = FVG_OK;
+    rc = rc ? rc : checked_alloc(c, &c->U, (size_t)NV * c->nc);
+    rc = rc ? rc : checked_alloc(c, &c->Ut, (size_t)NV * c->nc);
+    rc = rc ? rc : checked_alloc(c, &c->Rb, (size_t)NV * c->nc);
+    c->fp32 = cfg.flux_precision == FVG_FLUX_FP32;
+    const size_t fe = c->fp32 ? sizeof(float) : sizeof(double);
+    rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * c->nc, fe);
+    if (visc) rc = rc ? rc : checked_alloc(c, &c->G, (size_t)ND * ND * c->nc, fe);
+    if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
+    for (int d = 0; d < ND; ++d) {
+        rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d], fe);
+        if (visc) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d], fe);
+    }
+    rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
+    rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
+    if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
+    if (!rc && hipHostMalloc(&c->st_host, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipHostMalloc of the state mirror");
+    if (rc) {
+        const std::string m = c->err;
+        free_ctx(c);
+        return set_err(nullptr, rc, m);
+    }
+    std::memset(c->st_host, 0, sizeof(DevState));
+
+    // kernel parameters that do not change during the run
+    KP& p = c->base;
+    fill_kernel_params(cfg, p);
     p.U = c->U;
     p.W = c->W;
     p.G = c->G;
@@ -469,6 +514,8 @@ int fvg_residual(fvg_ctx* c, double half_dt, double* R, int ncomp, double* s_max
     int rc;
     DevState s = fresh_state(c);
     const bool hancock = half_dt > 0.0 && c->cfg.reconstruction != FVG_RECON_NONE;
+    if (hancock && c->fp32)   // the predictor kernels are FP64 only, and the primitive planes hold floats in this mode
+        return set_err(c, FVG_EUNSUPPORTED, "the MUSCL-Hancock predictor is not on the fp32-flux path");
     if (hancock && !c->HS) {
         if ((rc = checked_alloc(c, &c->HS, (size_t)2 * c->ND * c->NV * c->nc))) return rc;
         c->base.HS = c->HS;

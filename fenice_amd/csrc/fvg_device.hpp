@@ -26,6 +26,18 @@ __host__ __device__ constexpr int mom(int a) { return a == 0 ? 1 : (a == 1 ? 2 :
 template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 template <typename T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 
+// Division and square root of the flux arithmetic. FP64: the IEEE-rounded operations (the reference's bits).
+// FP32 (flux_precision = fp32): the hardware reciprocal and square root (about 1 ulp), two instructions instead of
+// the ten of a correctly rounded sequence; the error of the FP32 mode is measured against the FP64 path.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sq(double a) { return sqrt(a); }
+__device__ __forceinline__ float sq(float a) { return __builtin_amdgcn_sqrtf(a); }
+// the absolute floor of the WENO regularisation: 1e-300 in FP64 (Schemes.cpp:331, 343); the smallest power of ten
+// above FLT_MIN in FP32, where 1e-300 would round to zero and 0/0 would reach the weights of an all-zero stencil
+template <typename T> __device__ __forceinline__ T weno_floor() { return T(1e-300); }
+template <> __device__ __forceinline__ float weno_floor<float>() { return 1e-37f; }
+
 struct Eos {
     int stiff;
     double gamma, p_inf;
@@ -44,7 +56,7 @@ __device__ __forceinline__ void p2c(const Eos& e, const T* W, T* U) {
     } else {
         ke = T(0.5) * rho * (u * u + v * v);
     }
-    const T en = !e.stiff ? p / (g - T(1.0)) : (p + g * T(e.p_inf)) / (g - T(1.0));
+    const T en = !e.stiff ? dv(p, g - T(1.0)) : dv(p + g * T(e.p_inf), g - T(1.0));
     U[0] = rho;
     U[1] = rho * u;
     U[2] = rho * v;
@@ -56,7 +68,7 @@ template <int NV, typename T>
 __device__ __forceinline__ void c2p(const Eos& e, const T* U, T* W, T* inv_out) {
     const T g = T(e.gamma);
     const T rho = smax(U[0], T(kEosEps));
-    const T inv = T(1.0) / rho;
+    const T inv = dv(T(1.0), rho);
     if (inv_out) *inv_out = inv;
     const T u = U[1] * inv, v = U[2] * inv;
     T e_kin;
@@ -77,13 +89,13 @@ __device__ __forceinline__ void c2p(const Eos& e, const T* U, T* W, T* inv_out) 
 
 template <typename T>
 __device__ __forceinline__ T sound_speed(const Eos& e, T p, T rho) {
-    if (!e.stiff) return sqrt(T(e.gamma) * smax(p, T(kEosEps)) / smax(rho, T(kEosEps)));
-    return sqrt(T(e.gamma) * (p + T(e.p_inf)) / smax(rho, T(kEosEps)));
+    if (!e.stiff) return sq(dv(T(e.gamma) * smax(p, T(kEosEps)), smax(rho, T(kEosEps))));
+    return sq(dv(T(e.gamma) * (p + T(e.p_inf)), smax(rho, T(kEosEps))));
 }
 template <typename T>
 __device__ __forceinline__ T sound_speed_inv(const Eos& e, T p, T inv_rho) {
-    if (!e.stiff) return sqrt(T(e.gamma) * smax(p, T(kEosEps)) * inv_rho);
-    return sqrt(T(e.gamma) * (p + T(e.p_inf)) * inv_rho);
+    if (!e.stiff) return sq(T(e.gamma) * smax(p, T(kEosEps)) * inv_rho);
+    return sq(T(e.gamma) * (p + T(e.p_inf)) * inv_rho);
 }
 
 template <int NV, typename T>
@@ -127,13 +139,13 @@ __device__ __forceinline__ void hllc_core(const RiemannParams& rs, const T* WL, 
     const T S_R = smax(Vn_L + c_L, Vn_R + c_R);
     const T num = rho_R * Vn_R * (S_R - Vn_R) - rho_L * Vn_L * (S_L - Vn_L) + p_L - p_R;
     const T den = rho_R * (S_R - Vn_R) - rho_L * (S_L - Vn_L);
-    const T S_M = num / safe_denom(den);
+    const T S_M = dv(num, safe_denom(den));
     const T den_star_L = safe_denom(S_L - S_M);
     const T den_star_R = safe_denom(S_R - S_M);
     const T den_pL = safe_denom(S_L - Vn_L);
     const T den_pR = safe_denom(S_R - Vn_R);
-    const T fL = rho_L * (S_L - Vn_L) / den_star_L;
-    const T fR = rho_R * (S_R - Vn_R) / den_star_R;
+    const T fL = dv(rho_L * (S_L - Vn_L), den_star_L);
+    const T fR = dv(rho_R * (S_R - Vn_R), den_star_R);
     T UsL[NV], UsR[NV];
     UsL[0] = fL;
     UsR[0] = fR;
@@ -142,8 +154,8 @@ __device__ __forceinline__ void hllc_core(const RiemannParams& rs, const T* WL, 
         UsL[mom(a)] = fL * (a == axis ? S_M : WL[mom(a)]);
         UsR[mom(a)] = fR * (a == axis ? S_M : WR[mom(a)]);
     }
-    UsL[3] = fL * (E_L + (S_M - Vn_L) * (S_M + p_L / (rho_L * den_pL)));
-    UsR[3] = fR * (E_R + (S_M - Vn_R) * (S_M + p_R / (rho_R * den_pR)));
+    UsL[3] = fL * (E_L + (S_M - Vn_L) * (S_M + dv(p_L, rho_L * den_pL)));
+    UsR[3] = fR * (E_R + (S_M - Vn_R) * (S_M + dv(p_R, rho_R * den_pR)));
     T Fo[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) Fo[q] = T(0);
@@ -161,9 +173,9 @@ __device__ __forceinline__ void hllc_core(const RiemannParams& rs, const T* WL, 
         for (int q = 0; q < NV; ++q) Fo[q] = FR[q];
     if (rs.blend > 0.0) {
         const T blend = T(rs.blend);
-        const T s = fabs(p_R - p_L) / smin(p_L, p_R);
+        const T s = dv(fabs(p_R - p_L), smin(p_L, p_R));
         if (s > blend) {
-            const T phi = smin((s - blend) / blend, T(1.0));
+            const T phi = smin(dv(s - blend, blend), T(1.0));
             T Fh[NV];
             if (S_L >= T(0)) {
 #pragma unroll
@@ -172,7 +184,7 @@ __device__ __forceinline__ void hllc_core(const RiemannParams& rs, const T* WL, 
 #pragma unroll
                 for (int q = 0; q < NV; ++q) Fh[q] = FR[q];
             } else {
-                const T inv = T(1.0) / (S_R - S_L);
+                const T inv = dv(T(1.0), S_R - S_L);
 #pragma unroll
                 for (int q = 0; q < NV; ++q) Fh[q] = (S_R * FL[q] - S_L * FR[q] + S_L * S_R * (UR[q] - UL[q])) * inv;
             }
@@ -206,8 +218,8 @@ __device__ __forceinline__ void face_flux(const RiemannParams& rs, const T* UL, 
         ML2 = WL[1] * WL[1] + WL[2] * WL[2];
         MR2 = WR[1] * WR[1] + WR[2] * WR[2];
     }
-    const T M_L = sqrt(ML2) / (c_L + T(kMachEps));
-    const T M_R = sqrt(MR2) / (c_R + T(kMachEps));
+    const T M_L = dv(sq(ML2), c_L + T(kMachEps));
+    const T M_R = dv(sq(MR2), c_R + T(kMachEps));
     const T z = smin(smax(smax(M_L, M_R), T(1e-4)), T(1.0));
 #pragma unroll
     for (int a = 0; a < NV - 2; ++a) {
@@ -232,33 +244,33 @@ __device__ __forceinline__ T limiter(int l, T a, T b) {
     case 0: return T(0.5) * (sgn(a) + sgn(b)) * smin(fabs(a), fabs(b));
     case 1: {
         const T ab = a * b;
-        return (ab > T(0)) ? T(2.0) * ab / (a + b) : T(0.0);
+        return (ab > T(0)) ? dv(T(2.0) * ab, a + b) : T(0.0);
     }
     case 2:
         if (!(a * b > T(0))) return T(0.0);
         return smin(smin(T(2.0) * fabs(a), T(0.5) * fabs(a + b)), T(2.0) * fabs(b)) * sgn(a);
     default:
         if (!(a * b > T(0))) return T(0.0);
-        return fmin(fmin(T(2.0) * fabs(a), fabs(T(2.0) * a + b) / T(3.0)), T(2.0) * fabs(b)) * sgn(a);
+        return fmin(fmin(T(2.0) * fabs(a), dv(fabs(T(2.0) * a + b), T(3.0))), T(2.0) * fabs(b)) * sgn(a);
     }
 }
 
 template <typename T>
 __device__ __forceinline__ T weno3z(T c0, T c1, T c2, T eps_rel) {
-    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2) * T(1.0 / 3.0) + T(1e-300);
+    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2) * T(1.0 / 3.0) + weno_floor<T>();
     const T v0 = T(-0.5) * c0 + T(1.5) * c1, v1 = T(0.5) * c1 + T(0.5) * c2;
     const T b0 = (c1 - c0) * (c1 - c0), b1 = (c2 - c1) * (c2 - c1);
     const T tau = fabs(b0 - b1);
-    const T a0 = T(1.0 / 3.0) * (T(1.0) + tau / (b0 + eps)), a1 = T(2.0 / 3.0) * (T(1.0) + tau / (b1 + eps));
-    return (a0 * v0 + a1 * v1) / (a0 + a1);
+    const T a0 = T(1.0 / 3.0) * (T(1.0) + dv(tau, b0 + eps)), a1 = T(2.0 / 3.0) * (T(1.0) + dv(tau, b1 + eps));
+    return dv(a0 * v0 + a1 * v1, a0 + a1);
 }
 
 template <typename T>
 __device__ __forceinline__ T weno5z(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
-    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2) + T(1e-300);
-    const T p0 = (T(2.0) * c0 - T(7.0) * c1 + T(11.0) * c2) / T(6.0);
-    const T p1 = (-c1 + T(5.0) * c2 + T(2.0) * c3) / T(6.0);
-    const T p2 = (T(2.0) * c2 + T(5.0) * c3 - c4) / T(6.0);
+    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2) + weno_floor<T>();
+    const T p0 = dv(T(2.0) * c0 - T(7.0) * c1 + T(11.0) * c2, T(6.0));
+    const T p1 = dv(-c1 + T(5.0) * c2 + T(2.0) * c3, T(6.0));
+    const T p2 = dv(T(2.0) * c2 + T(5.0) * c3 - c4, T(6.0));
     const T s0 = c0 - T(2.0) * c1 + c2, t0 = c0 - T(4.0) * c1 + T(3.0) * c2;
     const T s1 = c1 - T(2.0) * c2 + c3, t1 = c1 - c3;
     const T s2 = c2 - T(2.0) * c3 + c4, t2 = T(3.0) * c2 - T(4.0) * c3 + c4;
@@ -266,9 +278,9 @@ __device__ __forceinline__ T weno5z(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
     const T b1 = T(13.0 / 12.0) * s1 * s1 + T(0.25) * t1 * t1;
     const T b2 = T(13.0 / 12.0) * s2 * s2 + T(0.25) * t2 * t2;
     const T tau = fabs(b0 - b2);
-    const T a0 = T(0.1) * (T(1.0) + tau / (b0 + eps)), a1 = T(0.6) * (T(1.0) + tau / (b1 + eps)),
-            a2 = T(0.3) * (T(1.0) + tau / (b2 + eps));
-    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+    const T a0 = T(0.1) * (T(1.0) + dv(tau, b0 + eps)), a1 = T(0.6) * (T(1.0) + dv(tau, b1 + eps)),
+            a2 = T(0.3) * (T(1.0) + dv(tau, b2 + eps));
+    return dv(a0 * p0 + a1 * p1 + a2 * p2, a0 + a1 + a2);
 }
 
 // ---- convective ghost state (Boundaries.cpp:106-153) ----

@@ -1,0 +1,12 @@
+// Host-side pieces shared by the contexts of the C ABI (fvg_api.hip) and the 1-D ensemble (fvg_batch.hip).
+#pragma once
+
+#include "../../include/fvg.h"
+#include "fvg_kernels.hpp"
+
+namespace fvg {
+
+// Mesh, numerics and viscous scalars of the kernels for a finalized configuration (buffers are left to the caller).
+void fill_kernel_params(const fvg_config& cfg, KP& p);
+
+}   // namespace fvg

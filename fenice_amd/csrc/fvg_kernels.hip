@@ -34,9 +34,10 @@ template <int NV> __device__ __forceinline__ void load_cons(const KP& p, long lo
 #pragma unroll
     for (int q = 0; q < NV; ++q) u[q] = p.U[q * p.nc + c];
 }
-template <int NV> __device__ __forceinline__ void load_prim(const KP& p, long long c, double* w) {
+template <int NV, typename T = double> __device__ __forceinline__ void load_prim(const KP& p, long long c, T* w) {
+    const T* W = (const T*)p.W;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) w[q] = p.W[q * p.nc + c];
+    for (int q = 0; q < NV; ++q) w[q] = W[q * p.nc + c];
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs of the chip, each with its own L2. The logical block of a
@@ -75,25 +76,27 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 // The padded primitive state of the reconstruction at the cell whose coordinate along d is t (the other two
 // from (i, j, k)): the cell itself, or outside the domain the periodic wrap, the ghost-slab layer
 // (fill_ghost_slab, Boundaries.cpp:202-228) or the edge cell (pad_primitives, Schemes.cpp:915-969).
-template <int NV>
-__device__ __forceinline__ void stencil_prim(const KP& p, int d, int i, int j, int k, int t, double* w) {
+template <int NV, typename T = double>
+__device__ __forceinline__ void stencil_prim(const KP& p, int d, int i, int j, int k, int t, T* w) {
     const int n = ncell(p, d);
     if (t >= 0 && t < n) {
-        load_prim<NV>(p, cidx_d(p, d, i, j, k, t), w);
+        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, t), w);
         return;
     }
     const int hi = t >= n, side = 2 * d + hi;
     const int h = halo_of(p, side);
     if (h == 1) {
-        load_prim<NV>(p, cidx_d(p, d, i, j, k, hi ? t - n : t + n), w);
+        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, hi ? t - n : t + n), w);
     } else if (h == 2) {
         const int l = hi ? t - n : -1 - t;
-        double ui[NV], ug[NV];
+        double ui[NV], ug[NV], wd[NV];
         load_cons<NV>(p, cidx_d(p, d, i, j, k, hi ? n - 1 - l : l), ui);
         ghost_state<NV>(p.rs.eos, p.bp, side, ui, ug);
-        c2p<NV>(p.rs.eos, ug, w, (double*)nullptr);
+        c2p<NV>(p.rs.eos, ug, wd, (double*)nullptr);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
     } else {
-        load_prim<NV>(p, cidx_d(p, d, i, j, k, t < 0 ? 0 : n - 1), w);
+        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, t < 0 ? 0 : n - 1), w);
     }
 }
 
@@ -213,7 +216,7 @@ __device__ __forceinline__ void visc_pad_prim(const KP& p, int i, int j, int k, 
 // k_cells: primitives; with viscosity the cell velocity gradients and the viscous speed bound
 // (Physics.cpp:44-76 2-D, 520-555 and 733-741 3-D; ViscousBound.hpp:16-24)
 // ------------------------------------------------------------------------------------------------------------
-template <int NV>
+template <int NV, typename T = double>
 __global__ __launch_bounds__(256) void k_cells(KP p) {
     const long long c = xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
@@ -221,8 +224,9 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
         double u[NV], w[NV];
         load_cons<NV>(p, c, u);
         c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
+        T* Wo = (T*)p.W;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) p.W[q * p.nc + c] = w[q];
+        for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = T(w[q]);
         if (p.viscous) {
             const int i = (int)(c % p.nx);
             const long long r = c / p.nx;
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             for (int a = 0; a < ND; ++a) {
                 const double ua = w[mom(a)];
 #pragma unroll
-                for (int b = 0; b < ND; ++b) p.G[(a * ND + b) * p.nc + c] = (grad[b][a + 1] - ua * grad[b][0]) / rho;
+                for (int b = 0; b < ND; ++b) ((T*)p.G)[(a * ND + b) * p.nc + c] = T((grad[b][a + 1] - ua * grad[b][0]) / rho);
             }
             const double g = p.rs.eos.gamma;
             const double nu = smax((4.0 / 3.0) * (p.mu / rho), (p.k_cond * (g - 1.0)) / rho) + 1e-12;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(256) void k_hancock(KP p) {
         const int j = (int)(r % p.ny), k = (int)(r / p.ny);
         double st[NS][NV];
         double w0[NV];
-        load_prim<NV>(p, c, w0);
+        load_prim<NV, double>(p, c, w0);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const int t = coord(d, i, j, k), n = ncell(p, d);
@@ -353,8 +357,8 @@ __global__ __launch_bounds__(256) void k_hancock(KP p) {
 // Cross derivative d u_a / d x_b on a face normal to D: the average of the cell gradients of the two cells xL, xR
 // (the opposite cell across a periodic seam), or on a physical boundary face with boundary_order = 2 the
 // tangential difference of the face values, ghost and cell layer averaged (Physics.cpp:614-628).
-template <int NV, int D>
-__device__ __forceinline__ double cross_term(const KP& p, bool phys, int i, int j, int k, int tg, int tc, long long xL, long long xR, int a,
+template <int NV, int D, typename T>
+__device__ __forceinline__ T cross_term(const KP& p, bool phys, int i, int j, int k, int tg, int tc, long long xL, long long xR, int a,
                                              int b) {
     constexpr int ND = NV - 2;
     if (phys) {
@@ -368,86 +372,100 @@ __device__ __forceinline__ double cross_term(const KP& p, bool phys, int i, int 
             visc_pad_prim<NV>(p, (D == 0 ? tc : i) + oi, (D == 1 ? tc : j) + oj, (D == 2 ? tc : k) + ok, wc);
             f[s] = 0.5 * (wg[mom(a)] + wc[mom(a)]);
         }
-        return (f[0] - f[1]) / (2.0 * hstep(p, b));
+        return T((f[0] - f[1]) / (2.0 * hstep(p, b)));
     }
-    const double* Gp = p.G + (long long)(a * ND + b) * p.nc;
-    return 0.5 * (Gp[xL] + Gp[xR]);
+    const T* Gp = (const T*)p.G + (long long)(a * ND + b) * p.nc;
+    return T(0.5) * (Gp[xL] + Gp[xR]);
 }
 
-template <int NV, int D>
-__device__ __forceinline__ void viscous_face(const KP& p, int i, int j, int k, int t, int n, double* out) {
+template <int NV, int D, typename T>
+__device__ __forceinline__ void viscous_face(const KP& p, int i, int j, int k, int t, int n, T* out) {
     constexpr int ND = NV - 2;
     const bool lo = t == 0, hi = t == n;
     const long long cL = cidx_d(p, D, i, j, k, lo ? 0 : t - 1), cR = cidx_d(p, D, i, j, k, hi ? n - 1 : t);
-    double vL[ND], vR[ND], TL, TR;
+    T vL[ND], vR[ND], TL, TR;
     {
-        double w[NV];
-        if (lo) visc_pad_prim<NV>(p, D == 0 ? -1 : i, D == 1 ? -1 : j, D == 2 ? -1 : k, w);
-        else load_prim<NV>(p, cL, w);
+        T w[NV];
+        if (lo) {
+            double wd[NV];
+            visc_pad_prim<NV>(p, D == 0 ? -1 : i, D == 1 ? -1 : j, D == 2 ? -1 : k, wd);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
+        } else {
+            load_prim<NV, T>(p, cL, w);
+        }
 #pragma unroll
         for (int a = 0; a < ND; ++a) vL[a] = w[mom(a)];
-        TL = w[3] / w[0];
-        if (hi) visc_pad_prim<NV>(p, D == 0 ? n : i, D == 1 ? n : j, D == 2 ? n : k, w);
-        else load_prim<NV>(p, cR, w);
+        TL = dv(w[3], w[0]);
+        if (hi) {
+            double wd[NV];
+            visc_pad_prim<NV>(p, D == 0 ? n : i, D == 1 ? n : j, D == 2 ? n : k, wd);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
+        } else {
+            load_prim<NV, T>(p, cR, w);
+        }
 #pragma unroll
         for (int a = 0; a < ND; ++a) vR[a] = w[mom(a)];
-        TR = w[3] / w[0];
+        TR = dv(w[3], w[0]);
     }
     const long long xL = (lo && vper_of(p, D)) ? cidx_d(p, D, i, j, k, n - 1) : cL;
     const long long xR = (hi && vper_of(p, D)) ? cidx_d(p, D, i, j, k, 0) : cR;
     const bool phys = p.bo2 && !vper_of(p, D) && (lo || hi);
-    const double h = hstep(p, D);
+    const T h = T(hstep(p, D));
     const int tg = lo ? -1 : n, tc = lo ? 0 : n - 1;   // the ghost and the cell layer of a boundary face
-#define CROSS(a, b) cross_term<NV, D>(p, phys, i, j, k, tg, tc, xL, xR, a, b)
-    const double mu = p.mu;
+#define CROSS(a, b) cross_term<NV, D, T>(p, phys, i, j, k, tg, tc, xL, xR, a, b)
+    const T mu = T(p.mu), kc = T(p.k_cond), MD = T(MU_DIAG), MO = T(MU_OFF), half = T(0.5);
     if constexpr (ND == 2) {
-        const double u_f = 0.5 * (vL[0] + vR[0]), v_f = 0.5 * (vL[1] + vR[1]);
+        const T u_f = half * (vL[0] + vR[0]), v_f = half * (vL[1] + vR[1]);
         if constexpr (D == 0) {
-            const double du_dx = (vR[0] - vL[0]) / h, dv_dx = (vR[1] - vL[1]) / h, dT_dx = (TR - TL) / h;
-            const double du_dy = CROSS(0, 1), dv_dy = CROSS(1, 1);
-            const double tau_xx = mu * (MU_DIAG * du_dx - MU_OFF * dv_dy);
-            const double tau_xy = mu * (du_dy + dv_dx);
-            const double q_x = -p.k_cond * dT_dx;
-            out[0] = tau_xx * p.dy;
-            out[1] = tau_xy * p.dy;
-            out[2] = (u_f * tau_xx + v_f * tau_xy - q_x) * p.dy;
+            const T du_dx = dv(vR[0] - vL[0], h), dv_dx = dv(vR[1] - vL[1], h), dT_dx = dv(TR - TL, h);
+            const T du_dy = CROSS(0, 1), dv_dy = CROSS(1, 1);
+            const T tau_xx = mu * (MD * du_dx - MO * dv_dy);
+            const T tau_xy = mu * (du_dy + dv_dx);
+            const T q_x = -kc * dT_dx;
+            const T ar = T(p.dy);
+            out[0] = tau_xx * ar;
+            out[1] = tau_xy * ar;
+            out[2] = (u_f * tau_xx + v_f * tau_xy - q_x) * ar;
         } else {
-            const double dv_dy = (vR[1] - vL[1]) / h, du_dy = (vR[0] - vL[0]) / h, dT_dy = (TR - TL) / h;
-            const double du_dx = CROSS(0, 0), dv_dx = CROSS(1, 0);
-            const double tau_yy = mu * (MU_DIAG * dv_dy - MU_OFF * du_dx);
-            const double tau_yx = mu * (du_dy + dv_dx);
-            const double q_y = -p.k_cond * dT_dy;
-            out[0] = tau_yx * p.dx;
-            out[1] = tau_yy * p.dx;
-            out[2] = (u_f * tau_yx + v_f * tau_yy - q_y) * p.dx;
+            const T dv_dy = dv(vR[1] - vL[1], h), du_dy = dv(vR[0] - vL[0], h), dT_dy = dv(TR - TL, h);
+            const T du_dx = CROSS(0, 0), dv_dx = CROSS(1, 0);
+            const T tau_yy = mu * (MD * dv_dy - MO * du_dx);
+            const T tau_yx = mu * (du_dy + dv_dx);
+            const T q_y = -kc * dT_dy;
+            const T ar = T(p.dx);
+            out[0] = tau_yx * ar;
+            out[1] = tau_yy * ar;
+            out[2] = (u_f * tau_yx + v_f * tau_yy - q_y) * ar;
         }
     } else {
-        double Gn[3], vf[3];
+        T Gn[3], vf[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            Gn[a] = (vR[a] - vL[a]) / h;
-            vf[a] = 0.5 * (vL[a] + vR[a]);
+            Gn[a] = dv(vR[a] - vL[a], h);
+            vf[a] = half * (vL[a] + vR[a]);
         }
-        const double dT = (TR - TL) / h;
-        double tau[3];
+        const T dT = dv(TR - TL, h);
+        T tau[3];
         if constexpr (D == 0) {
-            const double g01 = CROSS(0, 1), g11 = CROSS(1, 1), g22 = CROSS(2, 2), g02 = CROSS(0, 2);
-            tau[0] = mu * (MU_DIAG * Gn[0] - MU_OFF * ((0.0 + g11) + g22));
+            const T g01 = CROSS(0, 1), g11 = CROSS(1, 1), g22 = CROSS(2, 2), g02 = CROSS(0, 2);
+            tau[0] = mu * (MD * Gn[0] - MO * ((T(0.0) + g11) + g22));
             tau[1] = mu * (Gn[1] + g01);
             tau[2] = mu * (Gn[2] + g02);
         } else if constexpr (D == 1) {
-            const double g00 = CROSS(0, 0), g10 = CROSS(1, 0), g22 = CROSS(2, 2), g12 = CROSS(1, 2);
+            const T g00 = CROSS(0, 0), g10 = CROSS(1, 0), g22 = CROSS(2, 2), g12 = CROSS(1, 2);
             tau[0] = mu * (Gn[0] + g10);
-            tau[1] = mu * (MU_DIAG * Gn[1] - MU_OFF * ((0.0 + g00) + g22));
+            tau[1] = mu * (MD * Gn[1] - MO * ((T(0.0) + g00) + g22));
             tau[2] = mu * (Gn[2] + g12);
         } else {
-            const double g00 = CROSS(0, 0), g11 = CROSS(1, 1), g20 = CROSS(2, 0), g21 = CROSS(2, 1);
+            const T g00 = CROSS(0, 0), g11 = CROSS(1, 1), g20 = CROSS(2, 0), g21 = CROSS(2, 1);
             tau[0] = mu * (Gn[0] + g20);
             tau[1] = mu * (Gn[1] + g21);
-            tau[2] = mu * (MU_DIAG * Gn[2] - MU_OFF * ((0.0 + g00) + g11));
+            tau[2] = mu * (MD * Gn[2] - MO * ((T(0.0) + g00) + g11));
         }
-        const double q_d = -p.k_cond * dT;
-        const double area = D == 0 ? p.Ax : (D == 1 ? p.Ay : p.Az);
+        const T q_d = -kc * dT;
+        const T area = T(D == 0 ? p.Ax : (D == 1 ? p.Ay : p.Az));
         out[0] = tau[0] * area;
         out[1] = tau[1] * area;
         out[2] = (vf[0] * tau[0] + vf[1] * tau[1] + vf[2] * tau[2] - q_d) * area;
@@ -483,7 +501,13 @@ __device__ __forceinline__ void hancock_outside(const KP& p, int side, int i, in
     if (bc == 0 || bc == 1 || bc == 2) ghost_state<NV>(p.rs.eos, p.bp, side, in, out);
 }
 
-template <int NV, int D, int MODE>
+// double-precision boundary states into the flux arithmetic type
+template <int NV, typename T> __device__ __forceinline__ void to_t(const double* a, T* b) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) b[q] = T(a[q]);
+}
+
+template <int NV, int D, int MODE, typename T = double>
 __global__ __launch_bounds__(256) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
@@ -498,34 +522,38 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
         const bool lo = t == 0, hi = t == n;
         // the cells left and right of the face (their coordinate along D is t - 1 and t)
         const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
-        double UL[NV], UR[NV];
+        T UL[NV], UR[NV];
         if constexpr (MODE == MODE_FIRST_ORDER) {
-            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), UL);
-            else outside_first_order<NV>(p, D, 2 * D, i, j, k, UL);
-            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), UR);
-            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, UR);
+            double ul[NV], ur[NV];
+            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+            else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+            to_t<NV>(ul, UL);
+            to_t<NV>(ur, UR);
         } else if constexpr (MODE == MODE_PLAIN) {
             const bool upgrade = (t > 0 && t < n) || (lo && halo_of(p, 2 * D)) || (hi && halo_of(p, 2 * D + 1));
             if (upgrade) {
-                double wl[NV], wr[NV];
+                T wl[NV], wr[NV];
+                const T we = T(p.weno_eps);
                 if (p.recon == 3) {   // WENO5-Z on six cells t-3 .. t+2
-                    double c[6][NV];
+                    T c[6][NV];
 #pragma unroll
-                    for (int s = 0; s < 6; ++s) stencil_prim<NV>(p, D, i, j, k, t - 3 + s, c[s]);
+                    for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
 #pragma unroll
                     for (int q = 0; q < NV; ++q) {
-                        wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], p.weno_eps);
-                        wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], p.weno_eps);
+                        wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                        wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
                     }
                 } else {   // four cells t-2 .. t+1
-                    double c[4][NV];
+                    T c[4][NV];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) stencil_prim<NV>(p, D, i, j, k, t - 2 + s, c[s]);
+                    for (int s = 0; s < 4; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 2 + s, c[s]);
                     if (p.recon == 2) {
 #pragma unroll
                         for (int q = 0; q < NV; ++q) {
-                            wl[q] = weno3z(c[0][q], c[1][q], c[2][q], p.weno_eps);
-                            wr[q] = weno3z(c[3][q], c[2][q], c[1][q], p.weno_eps);
+                            wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
+                            wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
                         }
                     } else {
 #pragma unroll
@@ -538,12 +566,15 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
                 p2c<NV>(p.rs.eos, wl, UL);
                 p2c<NV>(p.rs.eos, wr, UR);
             } else {
-                if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), UL);
-                else outside_first_order<NV>(p, D, 2 * D, i, j, k, UL);
-                if (!hi) load_cons<NV>(p, cidx(p, i, j, k), UR);
-                else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, UR);
-                roundtrip<NV>(p.rs.eos, UL);
-                roundtrip<NV>(p.rs.eos, UR);
+                double ul[NV], ur[NV];
+                if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+                else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+                if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+                else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+                roundtrip<NV>(p.rs.eos, ul);
+                roundtrip<NV>(p.rs.eos, ur);
+                to_t<NV>(ul, UL);
+                to_t<NV>(ur, UR);
             }
         } else {   // MUSCL-Hancock: the predicted states of the cells on both sides (Schemes.cpp:2934-2988)
             if (!lo) {
@@ -559,17 +590,17 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
             if (lo) hancock_outside<NV, D>(p, 2 * D, i, j, k, t, UR, UL);
             if (hi) hancock_outside<NV, D>(p, 2 * D + 1, il, jl, kl, t, UL, UR);
         }
-        double F[NV], sm;
+        T F[NV], sm;
         face_flux<NV>(p.rs, UL, UR, D, F, sm);
-        if (sm > local) local = sm;
-        double* Fc = p.Fc[D];
+        if (double(sm) > local) local = double(sm);
+        T* Fc = (T*)p.Fc[D];
         const long long ps = p.nfa[D];
 #pragma unroll
         for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
         if (p.viscous) {
-            double v[NV - 1];
-            viscous_face<NV, D>(p, i, j, k, t, n, v);
-            double* Fv = p.Fv[D];
+            T v[NV - 1];
+            viscous_face<NV, D, T>(p, i, j, k, t, n, v);
+            T* Fv = (T*)p.Fv[D];
 #pragma unroll
             for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
         }
@@ -582,7 +613,7 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
 // (Physics.cpp:196-204, 746-752) and the stage combination of the time scheme (TimeScheme.cpp:27-59, 150-156,
 // 213-219), one cell per thread.
 // ------------------------------------------------------------------------------------------------------------
-template <int NV, int UPD>
+template <int NV, int UPD, typename T = double>
 __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
     const long long c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= p.nc || p.st->done) return;
@@ -596,11 +627,11 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
     double R[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        const double* Fx = p.Fc[0] + q * p.nfa[0];
-        const double* Fy = p.Fc[1] + q * p.nfa[1];
+        const T* Fx = (const T*)p.Fc[0] + q * p.nfa[0];
+        const T* Fy = (const T*)p.Fc[1] + q * p.nfa[1];
         const double fl = Fx[fx], fr = Fx[fx + 1], fb = Fy[fy], ft = Fy[fyt];
         if constexpr (NV == 5) {
-            const double* Fz = p.Fc[2] + q * p.nfa[2];
+            const T* Fz = (const T*)p.Fc[2] + q * p.nfa[2];
             const double ff = Fz[fz], fk = Fz[fzk];
             R[q] = (((0.0 - (fr * p.Ax - fl * p.Ax)) - (ft * p.Ay - fb * p.Ay)) - (fk * p.Az - ff * p.Az)) * p.inv_vol;
         } else {
@@ -611,13 +642,13 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
         R[0] = R[0] + 0.0;   // the zero viscous flux of the density slot
 #pragma unroll
         for (int q = 1; q < NV; ++q) {
-            const double* Fx = p.Fv[0] + (q - 1) * p.nfa[0];
-            const double* Fy = p.Fv[1] + (q - 1) * p.nfa[1];
+            const T* Fx = (const T*)p.Fv[0] + (q - 1) * p.nfa[0];
+            const T* Fy = (const T*)p.Fv[1] + (q - 1) * p.nfa[1];
             const double fl = Fx[fx], fr = Fx[fx + 1], fb = Fy[fy], ft = Fy[fyt];
             double v;
             if constexpr (NV == 5) {
-                const double* Fz = p.Fv[2] + (q - 1) * p.nfa[2];
-                v = (((0.0 + (fr - fl)) + (ft - fb)) + (Fz[fzk] - Fz[fz])) * p.inv_vol;
+                const T* Fz = (const T*)p.Fv[2] + (q - 1) * p.nfa[2];
+                v = (((0.0 + (fr - fl)) + (ft - fb)) + (double(Fz[fzk]) - double(Fz[fz]))) * p.inv_vol;
             } else {
                 v = ((0.0 + (fr - fl)) + (ft - fb)) * p.inv_vol;
             }
@@ -773,25 +804,33 @@ __global__ __launch_bounds__(256) void k_riemann_batch(RiemannParams rs, const d
 // ------------------------------------------------------------------------------------------------------------
 // Explicit instantiations used by the host
 // ------------------------------------------------------------------------------------------------------------
-#define FVG_INST(NV)                                                                                                  \
-    template __global__ void k_cells<NV>(KP);                                                                        \
-    template __global__ void k_hancock<NV>(KP);                                                                      \
-    template __global__ void k_faces<NV, 0, MODE_PLAIN>(KP);                                                         \
-    template __global__ void k_faces<NV, 1, MODE_PLAIN>(KP);                                                         \
-    template __global__ void k_faces<NV, 0, MODE_HANCOCK>(KP);                                                       \
-    template __global__ void k_faces<NV, 1, MODE_HANCOCK>(KP);                                                       \
-    template __global__ void k_faces<NV, 0, MODE_FIRST_ORDER>(KP);                                                   \
-    template __global__ void k_faces<NV, 1, MODE_FIRST_ORDER>(KP);                                                   \
-    template __global__ void k_update<NV, UPD_RESIDUAL>(KP, const double*, const double*, double*);                 \
-    template __global__ void k_update<NV, UPD_AXPY>(KP, const double*, const double*, double*);                     \
-    template __global__ void k_update<NV, UPD_RK2_2>(KP, const double*, const double*, double*);                    \
-    template __global__ void k_update<NV, UPD_RK3_2>(KP, const double*, const double*, double*);                    \
-    template __global__ void k_update<NV, UPD_RK3_3>(KP, const double*, const double*, double*);                    \
-    template __global__ void k_totals_partial<NV>(const double*, long long, double*);
-FVG_INST(4)
-FVG_INST(5)
-template __global__ void k_faces<5, 2, MODE_PLAIN>(KP);
-template __global__ void k_faces<5, 2, MODE_HANCOCK>(KP);
-template __global__ void k_faces<5, 2, MODE_FIRST_ORDER>(KP);
+#define FVG_INST_T(NV, T)                                                                                            \
+    template __global__ void k_cells<NV, T>(KP);                                                                     \
+    template __global__ void k_faces<NV, 0, MODE_PLAIN, T>(KP);                                                      \
+    template __global__ void k_faces<NV, 1, MODE_PLAIN, T>(KP);                                                      \
+    template __global__ void k_faces<NV, 0, MODE_FIRST_ORDER, T>(KP);                                                \
+    template __global__ void k_faces<NV, 1, MODE_FIRST_ORDER, T>(KP);                                                \
+    template __global__ void k_update<NV, UPD_RESIDUAL, T>(KP, const double*, const double*, double*);              \
+    template __global__ void k_update<NV, UPD_AXPY, T>(KP, const double*, const double*, double*);                  \
+    template __global__ void k_update<NV, UPD_RK2_2, T>(KP, const double*, const double*, double*);                 \
+    template __global__ void k_update<NV, UPD_RK3_2, T>(KP, const double*, const double*, double*);                 \
+    template __global__ void k_update<NV, UPD_RK3_3, T>(KP, const double*, const double*, double*);
+FVG_INST_T(4, double)
+FVG_INST_T(5, double)
+FVG_INST_T(4, float)
+FVG_INST_T(5, float)
+template __global__ void k_faces<5, 2, MODE_PLAIN, double>(KP);
+template __global__ void k_faces<5, 2, MODE_FIRST_ORDER, double>(KP);
+template __global__ void k_faces<5, 2, MODE_PLAIN, float>(KP);
+template __global__ void k_faces<5, 2, MODE_FIRST_ORDER, float>(KP);
+template __global__ void k_faces<5, 2, MODE_HANCOCK, double>(KP);
+template __global__ void k_hancock<4>(KP);
+template __global__ void k_hancock<5>(KP);
+template __global__ void k_faces<4, 0, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<4, 1, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<5, 0, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<5, 1, MODE_HANCOCK, double>(KP);
+template __global__ void k_totals_partial<4>(const double*, long long, double*);
+template __global__ void k_totals_partial<5>(const double*, long long, double*);
 
 }   // namespace fvg

@@ -43,12 +43,13 @@ struct KP {
     double mu, k_cond, inv_h2;
     int vper[3];    // periodic axes of the viscous stencil
     // buffers (plane sets)
-    const double* U;    // stage input state
-    double* W;          // primitives of U
-    double* G;          // viscous cell gradients (3-D: 9 planes G[a][b] at 3a+b; 2-D: dudx, dudy, dvdx, dvdy)
-    double* HS;         // MUSCL-Hancock predicted states: (2*ndim) * NV planes, state s = 2*axis + high
-    double* Fc[3];      // convective face flux per axis, NV planes each
-    double* Fv[3];      // viscous face flux per axis (3-D: slots 1..4; 2-D: slots 1..3)
+    // W, G, Fc and Fv hold doubles in the FP64 mode and floats in the FP32-flux mode (flux_precision)
+    const double* U;    // stage input state (always FP64)
+    void* W;            // primitives of U
+    void* G;            // viscous cell gradients (3-D: 9 planes G[a][b] at 3a+b; 2-D: dudx, dudy, dvdx, dvdy)
+    double* HS;         // MUSCL-Hancock predicted states: (2*ndim) * NV planes, state s = 2*axis + high (FP64 only)
+    void* Fc[3];        // convective face flux per axis, NV planes each
+    void* Fv[3];        // viscous face flux per axis (3-D: slots 1..4; 2-D: slots 1..3)
     long long nfa[3];   // faces per axis (plane stride of Fc / Fv)
     DevState* st;
 };

@@ -8,7 +8,8 @@ import sys
 import numpy as np
 import pytest
 
-from cases import golden_names, load_golden, matrix, state_for, base_keys, smooth_state
+from cases import (VORTEX_DIVERGES_AT, VORTEX_HANCOCK_256, base_keys, golden_names, load_golden, matrix, smooth_state,
+                   state_for, vortex_state)
 from fenice_amd import fvg
 from fenice_amd.config import Config, EOS_STIFFENED, HLLC, LOWMACH_HLLC
 from riemann_pool import HASHES, fingerprint, pool
@@ -165,3 +166,26 @@ def test_large_run_is_deterministic_and_conservative():
     tot1 = out[0][1]
     assert abs(tot1[0] - tot0[0]) <= 1e-13 * abs(tot0[0])
     assert abs(tot1[2] - tot0[2]) <= 1e-13 * abs(tot0[2])
+
+
+def test_vortex_hancock_cfl08_per_step_and_divergence(oracle):
+    # the isentropic vortex at 256^2, MUSCL-Hancock at CFL 0.8: bit-identical after each of steps 1-64, and the
+    # device reports the divergence at step 65 exactly where the oracle (and the reference) do
+    say("vortex hancock cfl 0.8")
+    keys = VORTEX_HANCOCK_256
+    cfg = fvg.config_from_keys(keys)
+    U = vortex_state(256)
+    Uo = U.copy()
+    t, lag = 0.0, 0.0
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        for step in range(1, VORTEX_DIVERGES_AT):
+            r = s.step(t, 1e9)
+            dto, so, lag, ok = oracle.step(cfg, Uo, t, 1e9, 0.0, lag)
+            assert ok and r.dt == dto and r.s_max == so, step
+            assert same_bits(s.download(), Uo), step
+            t += r.dt
+        assert s.lagged_speed == lag
+        with pytest.raises(fvg.Diverged):
+            s.step(t, 1e9)
+    assert not oracle.step(cfg, Uo, t, 1e9, 0.0, lag)[3]

@@ -3,7 +3,7 @@ predictor and whole steps, bit for bit, over the configuration matrix."""
 import numpy as np
 import pytest
 
-from cases import matrix, state_for
+from cases import VORTEX_DIVERGES_AT, VORTEX_HANCOCK_256, matrix, state_for, vortex_state
 from fenice_amd.fvg import config_from_keys
 
 MATRIX = matrix()
@@ -31,3 +31,22 @@ def test_oracle_equals_reference(oracle, ref, name, keys, kind):
         assert ok and dtr == dto and sr == so
         t += dtr
     assert np.array_equal(run.get_state().view(np.uint64), Uo.view(np.uint64))
+
+
+def test_vortex_hancock_cfl08_diverges_at_65(oracle, ref):
+    # every step of the unstable MUSCL-Hancock vortex run is bit-identical, and both sides reach the same step at
+    # which t + dt == t (the reference's check_progress, SimulationInternal.hpp:259-267)
+    keys = VORTEX_HANCOCK_256
+    run = ref.create("Isentropic Vortex", **keys)
+    cfg = config_from_keys(keys)
+    U = vortex_state(256)
+    run.set_state(U)
+    t, lag = 0.0, 0.0
+    for step in range(1, VORTEX_DIVERGES_AT + 1):
+        dtr, sr = run.step(t, 1e9)
+        dto, so, lag, ok = oracle.step(cfg, U, t, 1e9, 0.0, lag)
+        assert dtr == dto and sr == so, step
+        assert np.array_equal(run.get_state().view(np.uint64), U.view(np.uint64)), step
+        progressed = np.isfinite(sr) and np.isfinite(dtr) and t + dtr > t
+        assert ok == progressed and progressed == (step < VORTEX_DIVERGES_AT), step
+        t += dtr
