@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -17,11 +18,11 @@ LIB = os.path.join(HERE, "libfvg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FVG_ARCH", "gfx950")
 
-SOURCES = ["fvg_kernels.hip", "fvg_api.hip", "fvg_config.cpp"]
+SOURCES = ["fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip", "fvg_config.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-I" + os.path.join(ROOT, "include")]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-fgpu-rdc" if False else "-fno-gpu-rdc"]
+          "-fno-gpu-rdc"]
 
 
 def _run(cmd):
@@ -33,20 +34,21 @@ def _run(cmd):
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(obj_dir, src + ".o")
         if src.endswith(".hip"):
-            cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
+            cmds.append([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj])
         else:
-            cmd = [HIPCC, *COMMON, "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+            cmds.append([HIPCC, *COMMON, "-c", path, "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]), flush=True)
+    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -17,6 +17,7 @@ LIM_MINMOD, LIM_VAN_LEER, LIM_MC, LIM_KOREN = range(4)
 TS_AUTO, TS_EULER, TS_SSPRK2, TS_SSPRK3, TS_HANCOCK = -1, 0, 1, 2, 3
 EOS_IDEAL, EOS_STIFFENED = 0, 1
 FLUX_FP64, FLUX_FP32 = 0, 1
+INDICATOR_NONE, INDICATOR_LEARNED = 0, 1
 
 # status codes
 OK, EINVAL, ELOGIC, EDIVERGED, EHIP, ENOMEM, EUNSUPPORTED = range(7)
@@ -35,6 +36,8 @@ class Config(C.Structure):
         ("use_viscous", C.c_int), ("mu", C.c_double), ("pr", C.c_double),
         ("t_end", C.c_double), ("dt_max", C.c_double), ("max_steps", C.c_int),
         ("flux_precision", C.c_int),
+        ("indicator", C.c_int), ("indicator_audit", C.c_int), ("indicator_label", C.c_double),
+        ("indicator_fn_limit", C.c_double),
     ]
 
     @staticmethod
@@ -54,6 +57,7 @@ class Config(C.Structure):
         c.use_viscous, c.mu, c.pr = 0, 1e-3, 0.72
         c.t_end, c.dt_max, c.max_steps = 0.005, 0.0, 0
         c.flux_precision = FLUX_FP64
+        c.indicator, c.indicator_audit, c.indicator_label, c.indicator_fn_limit = INDICATOR_NONE, 50, 0.05, 0.01
         return c
 
     @staticmethod

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fvg.h"
@@ -20,6 +21,10 @@ template <int NV> __global__ void k_hancock(KP);
 template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV> __global__ void k_totals_partial(const double*, long long, double*);
+template <int NV, int D> __global__ void k_faces_learned(KP);
+template <int NV, int D> __global__ void k_audit(KP);
+template <int NV, int D> __global__ void k_samples(KP, float*, float*);
+__global__ void k_indicator_control(KP, int);
 __global__ void k_totals_final(const double*, int, long long, double*);
 __global__ void k_control(KP, int, int);
 __global__ void k_riemann_batch(RiemannParams, const double*, const double*, const int*, long long, double*, double*);
@@ -30,6 +35,7 @@ using namespace fvg;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kGraphSteps = 4;   // steps per captured graph of fvg_advance
 constexpr int kTotalsBlocks = 1024;
 thread_local std::string g_create_error;
 
@@ -68,6 +74,11 @@ struct fvg_ctx {
     std::vector<Rec> recs;
     double fam_ms[FAM_N] = {};
     long long fam_n[FAM_N] = {};
+    bool learned = false;              // learned troubled-face indicator
+    MlpDev* mlp = nullptr;
+    IndState* ind = nullptr;
+    bool graphs = true;                // fvg_advance replays a captured graph of steps
+    hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
 };
 
 namespace {
@@ -137,11 +148,27 @@ KP make_kp(fvg_ctx* c, const double* Uin) {
 }
 
 template <int NV, typename T>
-int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode) {
+int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
     const KP p = make_kp(c, Uin);
     int rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
         return rc;
+    if constexpr (std::is_same<T, float>::value) {
+        if (mode == MODE_LEARNED) {
+#define LFACES(KERNEL, D) \
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+            if (audit) {
+                if ((rc = LFACES(k_audit, 0)) || (rc = LFACES(k_audit, 1))) return rc;
+                if constexpr (NV == 5)
+                    if ((rc = LFACES(k_audit, 2))) return rc;
+            }
+            if ((rc = LFACES(k_faces_learned, 0)) || (rc = LFACES(k_faces_learned, 1))) return rc;
+            if constexpr (NV == 5)
+                if ((rc = LFACES(k_faces_learned, 2))) return rc;
+#undef LFACES
+            return FVG_OK;
+        }
+    }
     if (mode == MODE_HANCOCK)
         if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
             return rc;
@@ -183,11 +210,17 @@ int enqueue_hancock_residual(fvg_ctx* c, const double* Uin) {
     return enqueue_hancock_faces<NV>(c, p);
 }
 
-int enqueue_residual(fvg_ctx* c, const double* Uin, int mode) {
+int enqueue_residual(fvg_ctx* c, const double* Uin, int mode, bool audit = false) {
     if (mode == MODE_HANCOCK)
         return c->NV == 5 ? enqueue_hancock_residual<5>(c, Uin) : enqueue_hancock_residual<4>(c, Uin);
-    if (c->fp32) return c->NV == 5 ? enqueue_residual_nv<5, float>(c, Uin, mode) : enqueue_residual_nv<4, float>(c, Uin, mode);
-    return c->NV == 5 ? enqueue_residual_nv<5, double>(c, Uin, mode) : enqueue_residual_nv<4, double>(c, Uin, mode);
+    if (c->fp32)
+        return c->NV == 5 ? enqueue_residual_nv<5, float>(c, Uin, mode, audit) : enqueue_residual_nv<4, float>(c, Uin, mode, audit);
+    return c->NV == 5 ? enqueue_residual_nv<5, double>(c, Uin, mode, audit) : enqueue_residual_nv<4, double>(c, Uin, mode, audit);
+}
+
+int enqueue_indicator(fvg_ctx* c, int op) {
+    const KP p = c->base;
+    return launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_indicator_control, dim3(1), dim3(1), 0, c->stream, p, op); });
 }
 
 int enqueue_control(fvg_ctx* c, int op, int arg) {
@@ -214,7 +247,10 @@ int enqueue_update(fvg_ctx* c, int upd, const double* U0, const double* Ut, doub
     return c->NV == 5 ? enqueue_update_nv<5, double>(c, upd, U0, Ut, out) : enqueue_update_nv<4, double>(c, upd, U0, Ut, out);
 }
 
-int plain_mode(const fvg_ctx* c) { return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN; }
+int plain_mode(const fvg_ctx* c) {
+    if (c->learned) return MODE_LEARNED;
+    return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN;
+}
 
 // One step of the configured scheme, enqueued (TimeScheme.cpp:110-319).
 int enqueue_step(fvg_ctx* c) {
@@ -235,24 +271,52 @@ int enqueue_step(fvg_ctx* c) {
         return FVG_OK;
     }
     const int mode = plain_mode(c);
-    if ((rc = enqueue_residual(c, c->U, mode))) return rc;
+    if (c->learned && (rc = enqueue_indicator(c, CTL_AUDIT_BEGIN))) return rc;
+    if ((rc = enqueue_residual(c, c->U, mode, c->learned))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE1_DT, ts))) return rc;
     if (ts == FVG_TS_EULER) {
         if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
-        return enqueue_control(c, CTL_END_RK, ts);
+        if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
+        return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->Ut))) return rc;
     if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
     if (ts == FVG_TS_SSPRK2) {
         if ((rc = enqueue_update(c, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
-        return enqueue_control(c, CTL_END_RK, ts);
+        if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
+        return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_RK3_2, c->U, c->Ut, c->Ut))) return rc;
     if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
     if ((rc = enqueue_update(c, UPD_RK3_3, c->U, c->Ut, c->U))) return rc;
-    return enqueue_control(c, CTL_END_RK, ts);
+    if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
+    return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
+}
+
+// kGraphSteps steps of the scheme captured into one graph, once per context: every kernel reads time, dt and the
+// done flag from the device state, so the same graph serves every later call, and steps after the run's end are
+// no-ops. The MUSCL-Hancock seed residual is never in the graph (the first step after a reset of the lagged speed
+// runs eagerly).
+int ensure_graph(fvg_ctx* c) {
+    if (c->graph) return FVG_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FVG_OK;
+    for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(c);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return rc ? rc : set_err(c, FVG_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    const hipError_t e2 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        c->graph = nullptr;
+        return set_err(c, FVG_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    }
+    return FVG_OK;
 }
 
 int write_state(fvg_ctx* c, const DevState& s) {
@@ -281,6 +345,9 @@ void free_ctx(fvg_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    hipFree(c->mlp);
+    hipFree(c->ind);
     void* bufs[] = {c->U, c->Ut, c->Rb, c->W, c->G, c->HS, c->Fc[0], c->Fc[1], c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -309,6 +376,42 @@ int checked_alloc(fvg_ctx* c, P** p, size_t n, size_t elem = sizeof(double)) {
 }   // namespace
 
 namespace fvg {
+
+static unsigned bf16_host(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// The network's parameter vector (W1 32x15, b1, W2 32x32, b2, w3, b3) as the lanes of a wavefront read it
+// (fvg_learned.hpp): b1 folded into column 15 of W1, the weight matrices as bfloat16 MFMA fragments.
+MlpDev pack_mlp(const float* P, float thr) {
+    MlpDev m{};
+    const float* W1 = P;
+    const float* b1 = P + 480;
+    const float* W2 = P + 512;
+    auto w1 = [&](int h, int k) { return k < 15 ? W1[h * 15 + k] : b1[h]; };
+    for (int l = 0; l < 64; ++l) {
+        const int row = l % 32, k0 = 4 * (l / 32);
+        for (int kc = 0; kc < 2; ++kc) {
+            unsigned long long v = 0;
+            for (int i = 0; i < 4; ++i) v |= (unsigned long long)bf16_host(w1(row, 8 * kc + k0 + i)) << (16 * i);
+            m.w1[kc][l] = v;
+        }
+        for (int kc = 0; kc < 4; ++kc) {
+            unsigned long long v = 0;
+            for (int i = 0; i < 4; ++i) v |= (unsigned long long)bf16_host(W2[row * 32 + 8 * kc + k0 + i]) << (16 * i);
+            m.w2[kc][l] = v;
+        }
+    }
+    for (int h = 0; h < 32; ++h) {
+        m.b2[h] = P[1536 + h];
+        m.w3[h] = P[1568 + h];
+    }
+    m.b3 = P[1600];
+    m.thr = thr;
+    return m;
+}
 
 // Mesh, numerics and viscous scalars of the kernels for a finalized configuration (buffers are set by the caller).
 void fill_kernel_params(const fvg_config& cfg, KP& p) {
@@ -438,6 +541,19 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     }
     rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
     rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
+    c->learned = cfg.indicator == FVG_INDICATOR_LEARNED;
+    if (c->learned) {
+        rc = rc ? rc : checked_alloc(c, &c->mlp, 1, sizeof(MlpDev));
+        rc = rc ? rc : checked_alloc(c, &c->ind, 1, sizeof(IndState));
+        if (!rc) {   // until weights are set every face is troubled (WENO5-Z everywhere)
+            MlpDev m{};
+            m.thr = -std::numeric_limits<float>::max();
+            IndState z{};
+            if (hipMemcpy(c->mlp, &m, sizeof m, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->ind, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess)
+                rc = set_err(c, FVG_EHIP, "hipMemcpy of the indicator state");
+        }
+    }
     if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
     if (!rc && hipHostMalloc(&c->st_host, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipHostMalloc of the state mirror");
     if (rc) {
@@ -460,6 +576,11 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         p.nfa[d] = c->nfa[d];
     }
     p.st = c->st;
+    p.mlp = c->mlp;
+    p.ind = c->ind;
+    p.ind_audit = cfg.indicator_audit;
+    p.ind_label = (float)cfg.indicator_label;
+    p.ind_fn_limit = cfg.indicator_fn_limit;
     DevState s = fresh_state(c);
     if ((rc = write_state(c, s)) || (rc = (hipStreamSynchronize(c->stream) == hipSuccess ? 0 : FVG_EHIP))) {
         const std::string m = c->err;
@@ -570,11 +691,22 @@ int fvg_advance(fvg_ctx* c, int max_steps, double* t, double t_end, double dt_ma
     DevState o{};
     long long enqueued = 0;
     const int chunk = 16;
+    const bool graphed = c->graphs && !c->timing;
     for (;;) {
         const long long left = max_steps > 0 ? max_steps - enqueued : chunk;
-        const int n = (int)std::min<long long>(chunk, left);
-        for (int k = 0; k < n; ++k)
+        int n = (int)std::min<long long>(chunk, left);
+        if (graphed && c->cfg.time_scheme == FVG_TS_HANCOCK && !c->lagged_positive) {
             if ((rc = enqueue_step(c))) return rc;
+            enqueued += 1;
+            n -= 1;
+        }
+        if (graphed) {
+            if ((rc = ensure_graph(c))) return rc;
+            for (int k = 0; k < n; k += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+        } else {
+            for (int k = 0; k < n; ++k)
+                if ((rc = enqueue_step(c))) return rc;
+        }
         enqueued += n;
         if ((rc = read_state(c, o))) return rc;
         if (o.done || o.diverged || (max_steps > 0 && enqueued >= max_steps)) break;
@@ -620,6 +752,79 @@ int fvg_conserved_totals(fvg_ctx* c, double out[3]) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(out, c->tot_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_indicator_set(fvg_ctx* c, const float* params, int nparam, float threshold) {
+    if (!c || !params) return set_err(c, FVG_EINVAL, "null argument");
+    if (!c->learned) return set_err(c, FVG_ELOGIC, "the context was not created with indicator = learned");
+    if (nparam != MLP_NPARAM) return set_err(c, FVG_EINVAL, "the network has " + std::to_string(MLP_NPARAM) + " parameters");
+    for (int k = 0; k < nparam; ++k)
+        if (!std::isfinite(params[k])) return set_err(c, FVG_EINVAL, "non-finite network parameter");
+    if (std::isnan(threshold)) return set_err(c, FVG_EINVAL, "threshold is NaN");
+    HIPCHK(c, hipSetDevice(c->device));
+    const MlpDev m = pack_mlp(params, threshold);
+    IndState z{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->mlp, &m, sizeof m, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->ind, &z, sizeof z, hipMemcpyHostToDevice));
+    return FVG_OK;
+}
+
+int fvg_indicator_stats(fvg_ctx* c, unsigned long long counts[8], int* off, int* audits) {
+    if (!c) return FVG_EINVAL;
+    if (!c->learned) return set_err(c, FVG_ELOGIC, "the context was not created with indicator = learned");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    IndState h{};
+    HIPCHK(c, hipMemcpy(&h, c->ind, sizeof h, hipMemcpyDeviceToHost));
+    if (counts)
+        for (int q = 0; q < 4; ++q) {
+            counts[q] = h.last[q];
+            counts[4 + q] = h.total[q];
+        }
+    if (off) *off = h.off;
+    if (audits) *audits = h.audits;
+    return FVG_OK;
+}
+
+int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long nfaces) {
+    if (!c || !X || !dep) return set_err(c, FVG_EINVAL, "null argument");
+    if (!c->fp32) return set_err(c, FVG_ELOGIC, "samples are taken from a context with flux_precision = fp32");
+    if (axis < 0 || axis >= c->ND) return set_err(c, FVG_EINVAL, "axis out of range");
+    if (nfaces != c->nfa[axis]) return set_err(c, FVG_EINVAL, "nfaces differs from the axis' face count");
+    HIPCHK(c, hipSetDevice(c->device));
+    float *dX = nullptr, *dd = nullptr;
+    if (hipMalloc(&dX, sizeof(float) * 16 * (size_t)nfaces) != hipSuccess) return set_err(c, FVG_ENOMEM, "hipMalloc of the samples");
+    if (hipMalloc(&dd, sizeof(float) * (size_t)nfaces) != hipSuccess) {
+        hipFree(dX);
+        return set_err(c, FVG_ENOMEM, "hipMalloc of the samples");
+    }
+    const KP p = make_kp(c, c->U);
+    const dim3 gc(grid_for(c->nc)), gf((unsigned)((nfaces + kBlock - 1) / kBlock)), b(kBlock);
+    if (c->NV == 5) {
+        hipLaunchKernelGGL((k_cells<5, float>), gc, b, 0, c->stream, p);
+        if (axis == 0) hipLaunchKernelGGL((k_samples<5, 0>), gf, b, 0, c->stream, p, dX, dd);
+        else if (axis == 1) hipLaunchKernelGGL((k_samples<5, 1>), gf, b, 0, c->stream, p, dX, dd);
+        else hipLaunchKernelGGL((k_samples<5, 2>), gf, b, 0, c->stream, p, dX, dd);
+    } else {
+        hipLaunchKernelGGL((k_cells<4, float>), gc, b, 0, c->stream, p);
+        if (axis == 0) hipLaunchKernelGGL((k_samples<4, 0>), gf, b, 0, c->stream, p, dX, dd);
+        else hipLaunchKernelGGL((k_samples<4, 1>), gf, b, 0, c->stream, p, dX, dd);
+    }
+    int rc = FVG_OK;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, FVG_EHIP, "sample kernels failed");
+    if (!rc && (hipMemcpy(X, dX, sizeof(float) * 16 * (size_t)nfaces, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(dep, dd, sizeof(float) * (size_t)nfaces, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = set_err(c, FVG_EHIP, "hipMemcpy of the samples");
+    hipFree(dX);
+    hipFree(dd);
+    return rc;
+}
+
+int fvg_set_graphs(fvg_ctx* c, int on) {
+    if (!c) return FVG_EINVAL;
+    c->graphs = on != 0;
     return FVG_OK;
 }
 

@@ -1,0 +1,652 @@
+// An ensemble of independent 1-D runs in one persistent kernel (include/fvg.h, fvg_batch_*).
+//
+// Instance i is the reference's 2-D run on an N_i x 1 mesh (periodic y, one row): nx = N_i, ny = 1, the scheme
+// and the x boundaries of the shared configuration. One workgroup owns one instance for its whole run: it loops
+// over the steps and stages of the time scheme itself (Simulation::run_loop, SimulationRun.cpp:161-191;
+// TimeScheme.cpp:110-245), with workgroup barriers between the passes of a stage and the stage's wave speed
+// reduced in LDS, so the whole ensemble is one launch. Instances may differ in size, time step and step count.
+//
+// Exactness against the reference's N x 1 run. The y-momentum of such a run is +0 in every cell at every step:
+// the ghost and face states of the x sides allowed here keep v = +-0, v enters the other components only through
+// v * v or products with zero, and an update adds +-0 to +0. So only (rho, rho u, E) are stored, and v = +0 is
+// put back before the same device arithmetic as the 2-D path (NV = 4) runs. The two y faces of a cell see the
+// cell itself on both sides (periodic, one row, no halo: the first-order branch of the face pass), so their
+// fluxes are equal and cancel exactly in the balance (x - (+0) = x), but their wave speed enters the stage's
+// maximum: it is computed per cell from the same face_flux call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fvg_host.hpp"
+
+namespace fvg {
+
+struct BInst {
+    long long c0, f0;        // first cell / first face of the instance in the planes
+    int n;                   // cells
+    int halo_lo, halo_hi;    // reconstruction halo of the x sides (0 none, 1 periodic wrap, 2 ghost slab)
+    int pad;
+    double Ax, inv_vol, Ldt;   // mesh scalars of the N x 1 mesh
+};
+
+struct BOut {
+    double t, s_last, dt_last;
+    int steps, diverged;
+};
+
+struct BP {
+    RiemannParams rs;
+    BcParams bp;
+    int recon, lim, scheme, max_steps;   // scheme: 0 explicit Euler, 1 SSP-RK2, 2 SSP-RK3
+    double weno_eps, cfl, t_end, dt_max;
+    long long nc, nf;   // plane strides: cells and faces of the whole ensemble
+    double* U;          // 3 planes (rho, rho u, E): the state
+    double* Ut;         // 3 planes: the stage state
+    double* W;          // 3 planes (rho, u, p): primitives of the stage input
+    double* F;          // 3 planes: x-face fluxes
+    const BInst* inst;
+    const int* order;   // workgroup -> instance (longest estimated run first)
+    BOut* out;
+};
+
+// the state of cell i of the instance as the 2-D path holds it: (rho, rho u, rho v = +0, E)
+__device__ __forceinline__ void bload_cons(const BP& b, const BInst& in, const double* Us, int i, double* u) {
+    const long long c = in.c0 + i;
+    u[0] = Us[c];
+    u[1] = Us[b.nc + c];
+    u[2] = 0.0;
+    u[3] = Us[2 * b.nc + c];
+}
+__device__ __forceinline__ void bload_prim(const BP& b, const BInst& in, int i, double* w) {
+    const long long c = in.c0 + i;
+    w[0] = b.W[c];
+    w[1] = b.W[b.nc + c];
+    w[2] = 0.0;
+    w[3] = b.W[2 * b.nc + c];
+}
+
+// the padded primitive state at x index t (stencil_prim of the 2-D path, one row)
+__device__ __forceinline__ void bstencil(const BP& b, const BInst& in, const double* Us, int t, double* w) {
+    const int n = in.n;
+    if (t >= 0 && t < n) {
+        bload_prim(b, in, t, w);
+        return;
+    }
+    const int hi = t >= n;
+    const int h = hi ? in.halo_hi : in.halo_lo;
+    if (h == 1) {
+        bload_prim(b, in, hi ? t - n : t + n, w);
+    } else if (h == 2) {
+        const int l = hi ? t - n : -1 - t;
+        double ui[4], ug[4];
+        bload_cons(b, in, Us, hi ? n - 1 - l : l, ui);
+        ghost_state<4>(b.rs.eos, b.bp, hi, ui, ug);
+        c2p<4>(b.rs.eos, ug, w, (double*)nullptr);
+    } else {
+        bload_prim(b, in, t < 0 ? 0 : n - 1, w);
+    }
+}
+
+// the first-order state beyond x side `side` of its boundary cell (outside_first_order)
+__device__ __forceinline__ void boutside(const BP& b, const BInst& in, const double* Us, int side, double* ug) {
+    if (bc_of(b.bp, side) == 7) {
+        bload_cons(b, in, Us, side ? 0 : in.n - 1, ug);
+        return;
+    }
+    double ui[4];
+    bload_cons(b, in, Us, side ? in.n - 1 : 0, ui);
+    ghost_state<4>(b.rs.eos, b.bp, side, ui, ug);
+}
+
+// face f (between cells f - 1 and f) of the instance: the reconstruction (RECON = fvg_recon) and Riemann solve
+// of the face pass
+template <int RECON>
+__device__ __forceinline__ void bface(const BP& b, const BInst& in, const double* Us, int f, double* F, double& sm) {
+    const int n = in.n;
+    const bool lo = f == 0, hi = f == n;
+    double UL[4], UR[4];
+    if constexpr (RECON == 0) {   // first order (MODE_FIRST_ORDER): cell states, no round trip
+        if (!lo) bload_cons(b, in, Us, f - 1, UL);
+        else boutside(b, in, Us, 0, UL);
+        if (!hi) bload_cons(b, in, Us, f, UR);
+        else boutside(b, in, Us, 1, UR);
+    } else if ((f > 0 && f < n) || (lo && in.halo_lo) || (hi && in.halo_hi)) {
+        double wl[4], wr[4];
+        const double we = b.weno_eps;
+        if constexpr (RECON == 3) {
+            double c[6][4];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) bstencil(b, in, Us, f - 3 + s, c[s]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+            }
+        } else {
+            double c[4][4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) bstencil(b, in, Us, f - 2 + s, c[s]);
+            if constexpr (RECON == 2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
+                    wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    wl[q] = c[1][q] + 0.5 * limiter(b.lim, c[1][q] - c[0][q], c[2][q] - c[1][q]);
+                    wr[q] = c[2][q] - 0.5 * limiter(b.lim, c[3][q] - c[2][q], c[2][q] - c[1][q]);
+                }
+            }
+        }
+        p2c<4>(b.rs.eos, wl, UL);
+        p2c<4>(b.rs.eos, wr, UR);
+    } else {   // boundary face without a halo: first-order states through the primitive round trip
+        if (!lo) bload_cons(b, in, Us, f - 1, UL);
+        else boutside(b, in, Us, 0, UL);
+        if (!hi) bload_cons(b, in, Us, f, UR);
+        else boutside(b, in, Us, 1, UR);
+        roundtrip<4>(b.rs.eos, UL);
+        roundtrip<4>(b.rs.eos, UR);
+    }
+    face_flux<4>(b.rs, UL, UR, 0, F, sm);
+}
+
+// wave speed of the two (equal) y faces of cell i
+template <int RECON>
+__device__ __forceinline__ double by_speed(const BP& b, const BInst& in, const double* Us, int i) {
+    double u[4], F[4], sm;
+    bload_cons(b, in, Us, i, u);
+    if constexpr (RECON != 0) roundtrip<4>(b.rs.eos, u);
+    face_flux<4>(b.rs, u, u, 1, F, sm);
+    return sm;
+}
+
+// maximum over the workgroup of non-negative, non-NaN doubles; every thread gets it
+template <int BS>
+__device__ __forceinline__ double bmax_all(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double other = __shfl_xor(v, o, 64);
+        v = (other > v) ? other : v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double m = red[0];
+#pragma unroll
+    for (int w = 1; w < BS / 64; ++w) m = (red[w] > m) ? red[w] : m;
+    __syncthreads();
+    return m;
+}
+
+template <int BS, int RECON>
+__global__ __launch_bounds__(BS) void k_ensemble(BP b) {
+    __shared__ double red[BS / 64];
+    const int id = b.order[blockIdx.x];
+    const BInst in = b.inst[id];
+    const int n = in.n;
+    const int nst = b.scheme + 1;
+    double t = 0.0, dt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s_step = 0.0;
+    int steps = 0, diverged = 0;
+    while (t < b.t_end && steps < b.max_steps) {
+        for (int st = 1; st <= nst; ++st) {
+            const double* Us = st == 1 ? b.U : b.Ut;
+            // primitives of the stage input (k_cells)
+            for (int i = threadIdx.x; i < n; i += BS) {
+                double u[4], w[4];
+                bload_cons(b, in, Us, i, u);
+                c2p<4>(b.rs.eos, u, w, (double*)nullptr);
+                const long long c = in.c0 + i;
+                b.W[c] = w[0];
+                b.W[b.nc + c] = w[1];
+                b.W[2 * b.nc + c] = w[3];
+            }
+            __syncthreads();
+            // x faces and the y-face wave speed of each cell (k_faces)
+            double local = 0.0;
+            for (int f = threadIdx.x; f <= n; f += BS) {
+                double F[4], sm;
+                bface<RECON>(b, in, Us, f, F, sm);
+                if (sm > local) local = sm;
+                const long long x = in.f0 + f;
+                b.F[x] = F[0];
+                b.F[b.nf + x] = F[1];
+                b.F[2 * b.nf + x] = F[3];
+                if (f < n) {
+                    const double sy = by_speed<RECON>(b, in, Us, f);
+                    if (sy > local) local = sy;
+                }
+            }
+            const double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            // the scalar part of the scheme (k_control)
+            if (st == 1) {
+                s1 = S;
+                dt = smin(b.cfl * in.Ldt / (S + kDtEps), b.t_end - t);
+                if (b.dt_max > 0.0) dt = smin(dt, b.dt_max);
+            } else if (st == 2) {
+                s2 = S;
+            } else {
+                s3 = S;
+            }
+            // balance and stage combination (k_update)
+            for (int i = threadIdx.x; i < n; i += BS) {
+                const long long c = in.c0 + i, x = in.f0 + i;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double* Fq = b.F + q * b.nf;
+                    const double fl = Fq[x], fr = Fq[x + 1];
+                    const double R = (0.0 - (fr * in.Ax - fl * in.Ax)) * in.inv_vol;
+                    const long long y = q * b.nc + c;
+                    if (b.scheme == 0) {
+                        b.U[y] = b.U[y] + dt * R;
+                    } else if (st == 1) {
+                        b.Ut[y] = b.U[y] + dt * R;
+                    } else if (b.scheme == 1) {
+                        b.U[y] = 0.5 * b.U[y] + 0.5 * (b.Ut[y] + dt * R);
+                    } else if (st == 2) {
+                        b.Ut[y] = 0.75 * b.U[y] + 0.25 * (b.Ut[y] + dt * R);
+                    } else {
+                        b.U[y] = (1.0 / 3.0) * b.U[y] + (2.0 / 3.0) * (b.Ut[y] + dt * R);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        s_step = b.scheme == 0 ? s1 : (b.scheme == 1 ? smax(s1, s2) : smax(s1, smax(s2, s3)));
+        steps += 1;
+        // check_progress (SimulationInternal.hpp:259-267)
+        if (!(isfinite(s_step) && isfinite(dt) && t + dt > t)) {
+            diverged = 1;
+            break;
+        }
+        t += dt;
+    }
+    if (threadIdx.x == 0) {
+        BOut o;
+        o.t = t;
+        o.s_last = s_step;
+        o.dt_last = dt;
+        o.steps = steps;
+        o.diverged = diverged;
+        b.out[id] = o;
+    }
+}
+
+
+}   // namespace fvg
+
+using namespace fvg;
+
+namespace {
+std::string g_batch_create_error;
+const int kBlocks[] = {64, 128, 256, 512, 1024};
+}   // namespace
+
+struct fvg_batch {
+    fvg_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ninst = 0;
+    long long nc = 0, nf = 0;
+    std::vector<int> n;
+    std::vector<BInst> inst;
+    std::vector<int> order;
+    BInst* d_inst = nullptr;
+    int* d_order = nullptr;
+    BOut* d_out = nullptr;
+    double *U = nullptr, *Ut = nullptr, *W = nullptr, *F = nullptr, *Usave = nullptr;
+    BP base{};
+    int block = 256;
+    bool uploaded = false;
+    double last_ms = 0.0;
+    std::string err;
+};
+
+namespace {
+
+int bset(fvg_batch* b, int code, const std::string& m) {
+    if (b) b->err = m;
+    else g_batch_create_error = m;
+    return code;
+}
+
+#define BCHK(b, call)                                                                                      \
+    do {                                                                                                   \
+        const hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) return bset(b, FVG_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void free_batch(fvg_batch* b) {
+    if (!b) return;
+    hipFree(b->d_inst);
+    hipFree(b->d_order);
+    hipFree(b->d_out);
+    hipFree(b->U);
+    hipFree(b->Ut);
+    hipFree(b->W);
+    hipFree(b->F);
+    hipFree(b->Usave);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+template <int BS>
+void launch_bs(fvg_batch* b, const BP& p) {
+    const dim3 g(b->ninst), t(BS);
+    switch (p.recon) {
+    case 0: hipLaunchKernelGGL((k_ensemble<BS, 0>), g, t, 0, b->stream, p); break;
+    case 1: hipLaunchKernelGGL((k_ensemble<BS, 1>), g, t, 0, b->stream, p); break;
+    case 2: hipLaunchKernelGGL((k_ensemble<BS, 2>), g, t, 0, b->stream, p); break;
+    default: hipLaunchKernelGGL((k_ensemble<BS, 3>), g, t, 0, b->stream, p); break;
+    }
+}
+
+int launch_ensemble(fvg_batch* b, const BP& p, int block) {
+    switch (block) {
+    case 64: launch_bs<64>(b, p); break;
+    case 128: launch_bs<128>(b, p); break;
+    case 256: launch_bs<256>(b, p); break;
+    case 512: launch_bs<512>(b, p); break;
+    default: launch_bs<1024>(b, p); break;
+    }
+    BCHK(b, hipGetLastError());
+    return FVG_OK;
+}
+
+// One timed launch over the uploaded state.
+int run_once(fvg_batch* b, double t_end, double dt_max, int max_steps, int block, double* ms) {
+    BP p = b->base;
+    p.t_end = t_end;
+    p.dt_max = dt_max;
+    p.max_steps = max_steps;
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return bset(b, FVG_EHIP, "hipEventCreate failed");
+    }
+    int rc = FVG_OK;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = bset(b, FVG_EHIP, "hipEventRecord failed");
+    if (!rc) rc = launch_ensemble(b, p, block);
+    if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = bset(b, FVG_EHIP, "hipEventRecord failed");
+    if (!rc && hipStreamSynchronize(b->stream) != hipSuccess) rc = bset(b, FVG_EHIP, "ensemble kernel failed");
+    float f = 0.0f;
+    if (!rc && hipEventElapsedTime(&f, e0, e1) != hipSuccess) rc = bset(b, FVG_EHIP, "hipEventElapsedTime failed");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms) *ms = f;
+    return rc;
+}
+
+std::string tune_key(const fvg_batch* b) {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) arch = prop.gcnArchName;
+    unsigned long long h = 1469598103934665603ull;   // FNV-1a over the instance sizes
+    for (int v : b->n) {
+        h ^= (unsigned long long)v;
+        h *= 1099511628211ull;
+    }
+    std::ostringstream s;
+    s << arch << "|" << b->ninst << "|" << b->nc << "|" << std::hex << h << std::dec << "|" << b->cfg.reconstruction << "|"
+      << b->cfg.limiter << "|" << b->cfg.riemann << "|" << b->cfg.time_scheme;
+    return s.str();
+}
+
+}   // namespace
+
+extern "C" {
+
+const char* fvg_batch_last_error(const fvg_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
+
+int fvg_batch_create(const fvg_config* cfg_in, int ninst, const int* ncells, int device, fvg_batch** out) {
+    if (!cfg_in || !ncells || !out) return bset(nullptr, FVG_EINVAL, "null argument");
+    *out = nullptr;
+    if (ninst <= 0) return bset(nullptr, FVG_EINVAL, "ninst must be positive");
+    fvg_config cfg = *cfg_in;
+    cfg.nx = ncells[0];
+    cfg.ny = 1;
+    cfg.nz = 1;
+    char msg[256] = {0};
+    if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return bset(nullptr, FVG_EINVAL, msg);
+    if (cfg.bc[2] != FVG_BC_PERIODIC || cfg.bc[3] != FVG_BC_PERIODIC)
+        return bset(nullptr, FVG_EUNSUPPORTED, "an ensemble instance is one periodic row: bc_ymin and bc_ymax must be periodic");
+    for (int s = 0; s < 2; ++s) {
+        const int bt = cfg.bc[s];
+        if (!(bt == FVG_BC_WALL || bt == FVG_BC_NO_SLIP || bt == FVG_BC_TRANSMISSIVE || bt == FVG_BC_OUTLET || bt == FVG_BC_PERIODIC))
+            return bset(nullptr, FVG_EUNSUPPORTED, "x boundary of an ensemble must be wall, no_slip, transmissive, outlet or periodic");
+    }
+    if (cfg.use_viscous) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble runs the Euler equations (use_viscous = false)");
+    if (cfg.time_scheme == FVG_TS_HANCOCK) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble runs explicit Euler or SSP Runge-Kutta schemes");
+    if (cfg.flux_precision != FVG_FLUX_FP64) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble computes in FP64");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bset(nullptr, FVG_EHIP, "no HIP device");
+    if (device < 0 || device >= ndev) return bset(nullptr, FVG_EINVAL, "device index out of range");
+
+    fvg_batch* b = new fvg_batch();
+    b->cfg = cfg;
+    b->device = device;
+    b->ninst = ninst;
+    b->n.assign(ncells, ncells + ninst);
+    b->inst.resize(ninst);
+    long long c0 = 0;
+    for (int i = 0; i < ninst; ++i) {
+        if (ncells[i] < 1 || ncells[i] > (1 << 26)) {
+            free_batch(b);
+            return bset(nullptr, FVG_EINVAL, "instance " + std::to_string(i) + ": cell count out of range");
+        }
+        fvg_config ci = cfg;
+        ci.nx = ncells[i];
+        KP kp{};
+        fill_kernel_params(ci, kp);
+        BInst& in = b->inst[i];
+        in.c0 = c0;
+        in.f0 = c0 + i;
+        in.n = ncells[i];
+        in.halo_lo = kp.halo[0];
+        in.halo_hi = kp.halo[1];
+        in.pad = 0;
+        in.Ax = kp.Ax;
+        in.inv_vol = kp.inv_vol;
+        in.Ldt = kp.Ldt;
+        c0 += ncells[i];
+    }
+    b->nc = c0;
+    b->nf = c0 + ninst;
+    b->order.resize(ninst);
+    for (int i = 0; i < ninst; ++i) b->order[i] = i;
+    auto fail = [&](int code, const std::string& m) {
+        free_batch(b);
+        return bset(nullptr, code, m);
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(FVG_EHIP, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(FVG_EHIP, "hipStreamCreate failed");
+    const size_t cb = sizeof(double) * 3 * (size_t)b->nc, fb = sizeof(double) * 3 * (size_t)b->nf;
+    if (hipMalloc(&b->U, cb) != hipSuccess || hipMalloc(&b->Ut, cb) != hipSuccess || hipMalloc(&b->W, cb) != hipSuccess ||
+        hipMalloc(&b->Usave, cb) != hipSuccess || hipMalloc(&b->F, fb) != hipSuccess ||
+        hipMalloc(&b->d_inst, sizeof(BInst) * ninst) != hipSuccess || hipMalloc(&b->d_order, sizeof(int) * ninst) != hipSuccess ||
+        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess)
+        return fail(FVG_ENOMEM, "hipMalloc of the ensemble buffers");
+    if (hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_order, b->order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(FVG_EHIP, "hipMemcpy of the instance table");
+
+    KP kp{};
+    fill_kernel_params(cfg, kp);
+    BP& p = b->base;
+    p.rs = kp.rs;
+    p.bp = kp.bp;
+    p.recon = cfg.reconstruction;
+    p.lim = cfg.limiter;
+    p.scheme = cfg.time_scheme == FVG_TS_EULER ? 0 : (cfg.time_scheme == FVG_TS_SSPRK2 ? 1 : 2);
+    p.weno_eps = cfg.weno_epsilon;
+    p.cfl = cfg.cfl;
+    p.nc = b->nc;
+    p.nf = b->nf;
+    p.U = b->U;
+    p.Ut = b->Ut;
+    p.W = b->W;
+    p.F = b->F;
+    p.inst = b->d_inst;
+    p.order = b->d_order;
+    p.out = b->d_out;
+    *out = b;
+    return FVG_OK;
+}
+
+int fvg_batch_destroy(fvg_batch* b) {
+    free_batch(b);
+    return FVG_OK;
+}
+
+int fvg_batch_upload(fvg_batch* b, const double* U, long long ncells, int ncomp) {
+    if (!b || !U) return bset(b, FVG_EINVAL, "null argument");
+    if (ncells != b->nc) return bset(b, FVG_EINVAL, "cell count differs from the ensemble's total");
+    if (ncomp != 3 && ncomp != 4 && ncomp != 5) return bset(b, FVG_EINVAL, "ncomp must be 3, 4 or 5");
+    // (rho, rho u, E) planes; the estimated cost of each instance orders the workgroups (longest first)
+    const int e = ncomp == 3 ? 2 : 3;
+    std::vector<double> h((size_t)3 * b->nc);
+    const double g = b->cfg.gamma, pinf = b->cfg.eos_type == FVG_EOS_STIFFENED ? b->cfg.p_inf : 0.0;
+    std::vector<double> cost(b->ninst);
+    for (int i = 0; i < b->ninst; ++i) {
+        const BInst& in = b->inst[i];
+        double smax0 = 0.0;
+        for (int k = 0; k < in.n; ++k) {
+            const long long c = in.c0 + k;
+            const double* s = U + (size_t)ncomp * c;
+            h[c] = s[0];
+            h[b->nc + c] = s[1];
+            h[2 * b->nc + c] = s[e];
+            const double rho = std::max(s[0], 1e-10), u = s[1] / rho;
+            const double p = (g - 1.0) * (s[e] - 0.5 * rho * u * u) - (pinf != 0.0 ? g * pinf : 0.0);
+            const double c2 = g * (std::max(p, 1e-10) + pinf) / rho;
+            const double sp = std::fabs(u) + std::sqrt(std::max(c2, 0.0));
+            if (std::isfinite(sp) && sp > smax0) smax0 = sp;
+        }
+        cost[i] = (double)in.n * in.n * (smax0 + 1e-3);
+    }
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipMemcpy(b->U, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->ninst, hipMemcpyHostToDevice));
+    b->uploaded = true;
+    return FVG_OK;
+}
+
+int fvg_batch_download(fvg_batch* b, double* U, long long ncells, int ncomp) {
+    if (!b || !U) return bset(b, FVG_EINVAL, "null argument");
+    if (ncells != b->nc) return bset(b, FVG_EINVAL, "cell count differs from the ensemble's total");
+    if (ncomp != 3 && ncomp != 4 && ncomp != 5) return bset(b, FVG_EINVAL, "ncomp must be 3, 4 or 5");
+    std::vector<double> h((size_t)3 * b->nc);
+    BCHK(b, hipSetDevice(b->device));
+    BCHK(b, hipMemcpy(h.data(), b->U, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (long long c = 0; c < b->nc; ++c) {
+        double* s = U + (size_t)ncomp * c;
+        s[0] = h[c];
+        s[1] = h[b->nc + c];
+        if (ncomp == 3) {
+            s[2] = h[2 * b->nc + c];
+        } else {
+            s[2] = 0.0;
+            s[3] = h[2 * b->nc + c];
+            if (ncomp == 5) s[4] = 0.0;
+        }
+    }
+    return FVG_OK;
+}
+
+int fvg_batch_run(fvg_batch* b, double t_end, double dt_max, int max_steps, double* t, int* steps, int* diverged, double* s_last,
+                  double* ms) {
+    if (!b) return bset(b, FVG_EINVAL, "null argument");
+    if (!b->uploaded) return bset(b, FVG_ELOGIC, "no state uploaded");
+    if (!(max_steps > 0)) return bset(b, FVG_EINVAL, "max_steps must be positive (it bounds the kernel's step loop)");
+    if (!std::isfinite(t_end) || !(t_end >= 0.0)) return bset(b, FVG_EINVAL, "t_end must be finite and non-negative");
+    BCHK(b, hipSetDevice(b->device));
+    double m = 0.0;
+    int rc = run_once(b, t_end, dt_max, max_steps, b->block, &m);
+    if (rc) return rc;
+    b->last_ms = m;
+    if (ms) *ms = m;
+    std::vector<BOut> o(b->ninst);
+    BCHK(b, hipMemcpy(o.data(), b->d_out, sizeof(BOut) * b->ninst, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ninst; ++i) {
+        if (t) t[i] = o[i].t;
+        if (steps) steps[i] = o[i].steps;
+        if (diverged) diverged[i] = o[i].diverged;
+        if (s_last) s_last[i] = o[i].s_last;
+    }
+    return FVG_OK;
+}
+
+int fvg_batch_set_block(fvg_batch* b, int block) {
+    if (!b) return bset(b, FVG_EINVAL, "null argument");
+    for (int k : kBlocks)
+        if (k == block) {
+            b->block = block;
+            return FVG_OK;
+        }
+    return bset(b, FVG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+}
+
+int fvg_batch_get_block(const fvg_batch* b, int* block) {
+    if (!b || !block) return FVG_EINVAL;
+    *block = b->block;
+    return FVG_OK;
+}
+
+int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cache_path, int* block, int* from_cache) {
+    if (!b) return bset(b, FVG_EINVAL, "null argument");
+    if (!b->uploaded) return bset(b, FVG_ELOGIC, "no state uploaded");
+    if (!(max_steps > 0) || !std::isfinite(t_probe) || !(t_probe > 0.0)) return bset(b, FVG_EINVAL, "t_probe and max_steps must be positive");
+    if (from_cache) *from_cache = 0;
+    const std::string key = tune_key(b);
+    if (cache_path && *cache_path) {
+        std::ifstream f(cache_path);
+        std::string line;
+        while (std::getline(f, line)) {
+            std::istringstream s(line);
+            std::string k;
+            int blk = 0;
+            double tms = 0.0;
+            if ((s >> k >> blk >> tms) && k == key && fvg_batch_set_block(b, blk) == FVG_OK) {
+                if (block) *block = blk;
+                if (from_cache) *from_cache = 1;
+                return FVG_OK;
+            }
+        }
+    }
+    BCHK(b, hipSetDevice(b->device));
+    const size_t cb = sizeof(double) * 3 * (size_t)b->nc;
+    BCHK(b, hipMemcpyAsync(b->Usave, b->U, cb, hipMemcpyDeviceToDevice, b->stream));
+    int best = b->block;
+    double best_ms = 1e300;
+    for (int k : kBlocks) {
+        BCHK(b, hipMemcpyAsync(b->U, b->Usave, cb, hipMemcpyDeviceToDevice, b->stream));
+        double m = 0.0;
+        const int rc = run_once(b, t_probe, 0.0, max_steps, k, &m);
+        if (rc) return rc;
+        if (m < best_ms) {
+            best_ms = m;
+            best = k;
+        }
+    }
+    BCHK(b, hipMemcpyAsync(b->U, b->Usave, cb, hipMemcpyDeviceToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->block = best;
+    if (block) *block = best;
+    if (cache_path && *cache_path) {
+        std::ofstream f(cache_path, std::ios::app);
+        if (f) f << key << " " << best << " " << best_ms << "\n";
+    }
+    return FVG_OK;
+}
+
+}   // extern "C"

@@ -82,6 +82,10 @@ void fvg_config_defaults(fvg_config* c) {
     c->dt_max = 0.0;
     c->max_steps = 0;
     c->flux_precision = FVG_FLUX_FP64;
+    c->indicator = FVG_INDICATOR_NONE;
+    c->indicator_audit = 50;
+    c->indicator_label = 0.05;
+    c->indicator_fn_limit = 0.01;
 }
 
 int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* err, size_t n) {
@@ -164,6 +168,14 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
         if (lv == "fp32" || lv == "float") return (c->flux_precision = FVG_FLUX_FP32), FVG_OK;
         return fail(err, n, "Unknown flux precision: '" + v + "' (use fp64 or fp32)");
     }
+    if (key == "indicator") {
+        if (lv == "none" || lv == "weno") return (c->indicator = FVG_INDICATOR_NONE), FVG_OK;
+        if (lv == "learned") return (c->indicator = FVG_INDICATOR_LEARNED), FVG_OK;
+        return fail(err, n, "Unknown indicator: '" + v + "' (use none or learned)");
+    }
+    if (key == "indicator_audit") return integer(c->indicator_audit);
+    if (key == "indicator_label") return num(c->indicator_label);
+    if (key == "indicator_fn_limit") return num(c->indicator_fn_limit);
     static const char* bc_keys[6] = {"bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax", "bc_zmin", "bc_zmax"};
     for (int s = 0; s < 6; ++s) {
         if (key != bc_keys[s]) continue;
@@ -195,7 +207,21 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
         if (a < 2 || c->nz > 1)
             if (lo != hi) return fail(err, n, "Periodic boundaries come in pairs");
     }
+    // the checks of SimParams::from on the keys of the path (Params.cpp:429-460), with its messages
+    if (!(c->weno_epsilon >= 0.0)) return fail(err, n, "weno_epsilon must be >= 0");
+    if ((c->reconstruction == FVG_RECON_WENO3 || c->reconstruction == FVG_RECON_WENO5) && c->time_scheme == FVG_TS_SSPRK2)
+        return fail(err, n, std::string("reconstruction = ") + (c->reconstruction == FVG_RECON_WENO3 ? "weno3" : "weno5") +
+                                " is not used with SSP Runge-Kutta 2 (use SSP Runge-Kutta 3)");
+    if (c->time_scheme == FVG_TS_HANCOCK && c->reconstruction != FVG_RECON_MUSCL)
+        return fail(err, n, "time_scheme = MUSCL-Hancock needs reconstruction = muscl");
     if (c->boundary_order != 1 && c->boundary_order != 2) return fail(err, n, "boundary_order must be 1 or 2");
+    if (c->indicator == FVG_INDICATOR_LEARNED &&
+        (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))
+        return fail(err, n, "indicator = learned needs reconstruction = weno5, flux_precision = fp32 and a Runge-Kutta scheme");
+    if (c->indicator_audit < 0 || !(c->indicator_label > 0.0) || !(c->indicator_fn_limit >= 0.0))
+        return fail(err, n, "indicator_audit must be >= 0, indicator_label > 0 and indicator_fn_limit >= 0");
+    // a first-order scheme has no reconstruction to extend to the boundary faces (Params.cpp:460)
+    if (c->boundary_order == 2 && c->reconstruction == FVG_RECON_NONE) c->boundary_order = 1;
     return FVG_OK;
 }
 

@@ -9,4 +9,7 @@ namespace fvg {
 // Mesh, numerics and viscous scalars of the kernels for a finalized configuration (buffers are left to the caller).
 void fill_kernel_params(const fvg_config& cfg, KP& p);
 
+// The network's parameter vector (W1 32x15, b1, W2 32x32, b2, w3, b3; 1601 floats) as the device reads it.
+MlpDev pack_mlp(const float* params, float threshold);
+
 }   // namespace fvg

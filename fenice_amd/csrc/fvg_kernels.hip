@@ -609,6 +609,162 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// Learned troubled-face indicator (FP32 fluxes, WENO5-Z; fvg_learned.hpp). Faces the network calls smooth take
+// the linear upwind-5 value; the others, and every face once the indicator is off, take WENO5-Z. Faces without a
+// full stencil keep their first-order states as in k_faces.
+// ------------------------------------------------------------------------------------------------------------
+template <int NV, int D>
+__device__ __forceinline__ void face_coords(const KP& p, long long f, int& i, int& j, int& k, int& t) {
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1);
+    i = (int)(f % ni);
+    const long long r = f / ni;
+    j = (int)(r % nj);
+    k = (int)(r / nj);
+    t = coord(D, i, j, k);
+}
+
+template <int NV, int D>
+__device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
+    const int n = ncell(p, D);
+    return (t > 0 && t < n) || (t == 0 && halo_of(p, 2 * D)) || (t == n && halo_of(p, 2 * D + 1));
+}
+
+template <int NV, int D>
+__global__ __launch_bounds__(256) void k_faces_learned(KP p) {
+    if (p.st->done) return;   // uniform: the whole grid leaves
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
+    const long long nf = (long long)ni * nj * nk;
+    const long long f = xcd_block() * blockDim.x + threadIdx.x;
+    const bool active = f < nf;
+    int i = 0, j = 0, k = 0, t = 0;
+    if (active) face_coords<NV, D>(p, f, i, j, k, t);
+    const bool upgrade = active && face_upgrade<NV, D>(p, t);
+    float c[6][NV];
+    float x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = 0.0f;
+    if (upgrade) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+        face_features<NV>(c, float(p.weno_eps), x);
+    }
+    const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
+    double local = 0.0;
+    if (active) {
+        const int n = ncell(p, D);
+        const bool lo = t == 0, hi = t == n;
+        const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
+        float UL[NV], UR[NV];
+        if (upgrade) {
+            const bool smooth = z <= p.mlp->thr && !p.ind->off;
+            float wl[NV], wr[NV];
+            const float we = float(p.weno_eps);
+            if (smooth) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    wl[q] = upwind5(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q]);
+                    wr[q] = upwind5(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q]);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                    wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+                }
+            }
+            p2c<NV>(p.rs.eos, wl, UL);
+            p2c<NV>(p.rs.eos, wr, UR);
+        } else {
+            double ul[NV], ur[NV];
+            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+            else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+            roundtrip<NV>(p.rs.eos, ul);
+            roundtrip<NV>(p.rs.eos, ur);
+            to_t<NV>(ul, UL);
+            to_t<NV>(ur, UR);
+        }
+        float F[NV], sm;
+        face_flux<NV>(p.rs, UL, UR, D, F, sm);
+        if (double(sm) > local) local = double(sm);
+        float* Fc = (float*)p.Fc[D];
+        const long long ps = p.nfa[D];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
+        if (p.viscous) {
+            float v[NV - 1];
+            viscous_face<NV, D, float>(p, i, j, k, t, n, v);
+            float* Fv = (float*)p.Fv[D];
+#pragma unroll
+            for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
+        }
+    }
+    block_max_to(local, &p.st->s_conv);
+}
+
+// Audit of the indicator on the stage input of an audit step: the network's decision against the label (WENO5-Z
+// weight departure above ind_label) on every face with a full stencil; counts TP, FP, FN, TN.
+template <int NV, int D>
+__global__ __launch_bounds__(256) void k_audit(KP p) {
+    if (p.st->done || p.ind_audit <= 0 || (p.ind->steps % p.ind_audit) != 0) return;   // uniform
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
+    const long long nf = (long long)ni * nj * nk;
+    const long long f = xcd_block() * blockDim.x + threadIdx.x;
+    int i = 0, j = 0, k = 0, t = 0;
+    if (f < nf) face_coords<NV, D>(p, f, i, j, k, t);
+    const bool upgrade = f < nf && face_upgrade<NV, D>(p, t);
+    float c[6][NV];
+    float x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = 0.0f;
+    bool label = false;
+    if (upgrade) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+        face_features<NV>(c, float(p.weno_eps), x);
+        label = face_departure<NV, float>(c, float(p.weno_eps)) > p.ind_label;
+    }
+    const float z = mlp_logit_wave(p.mlp, x);
+    const bool pred = z > p.mlp->thr;
+    // counts of the wavefront, then one atomic per wavefront and class
+    const unsigned long long bal_tp = __ballot(upgrade && label && pred), bal_fp = __ballot(upgrade && !label && pred),
+                             bal_fn = __ballot(upgrade && label && !pred), bal_tn = __ballot(upgrade && !label && !pred);
+    if ((threadIdx.x & 63) == 0) {
+        if (bal_tp) atomicAdd(&p.ind->last[0], (unsigned long long)__popcll(bal_tp));
+        if (bal_fp) atomicAdd(&p.ind->last[1], (unsigned long long)__popcll(bal_fp));
+        if (bal_fn) atomicAdd(&p.ind->last[2], (unsigned long long)__popcll(bal_fn));
+        if (bal_tn) atomicAdd(&p.ind->last[3], (unsigned long long)__popcll(bal_tn));
+    }
+}
+
+// Training samples of one axis: the features (16 floats per face, the 16th = 1) and the label value of every face
+// of the current primitives; faces without a full stencil get label -1.
+template <int NV, int D>
+__global__ __launch_bounds__(256) void k_samples(KP p, float* __restrict__ X, float* __restrict__ dep) {
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
+    const long long nf = (long long)ni * nj * nk;
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    int i, j, k, t;
+    face_coords<NV, D>(p, f, i, j, k, t);
+    float x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = 0.0f;
+    float d = -1.0f;
+    if (face_upgrade<NV, D>(p, t)) {
+        float c[6][NV];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+        face_features<NV>(c, float(p.weno_eps), x);
+        d = face_departure<NV, float>(c, float(p.weno_eps));
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) X[f * 16 + q] = x[q];
+    dep[f] = d;
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // k_update: divergence of the face fluxes (Simulation.cpp:1022-1037, 1293-1310), the viscous balance added
 // (Physics.cpp:196-204, 746-752) and the stage combination of the time scheme (TimeScheme.cpp:27-59, 150-156,
 // 213-219), one cell per thread.
@@ -747,6 +903,30 @@ __global__ void k_control(KP p, int op, int arg) {
     }
 }
 
+// Indicator bookkeeping (one thread). Before a step's first stage: mark the step as taken (unless the run has
+// ended) and clear the audit counters of an audit step. After its last stage: add the counters to the totals,
+// switch the indicator off when it missed too many troubled faces, and count the step.
+__global__ void k_indicator_control(KP p, int op) {
+    IndState* d = p.ind;
+    const bool audit_step = p.ind_audit > 0 && (d->steps % p.ind_audit) == 0;
+    if (op == CTL_AUDIT_BEGIN) {
+        if (p.st->done) return;
+        d->pending = 1;
+        if (audit_step)
+            for (int q = 0; q < 4; ++q) d->last[q] = 0;
+        return;
+    }
+    if (!d->pending) return;
+    d->pending = 0;
+    if (audit_step) {
+        for (int q = 0; q < 4; ++q) d->total[q] += d->last[q];
+        d->audits += 1;
+        const double troubled = double(d->last[0] + d->last[2]);
+        if (troubled > 0.0 && double(d->last[2]) > p.ind_fn_limit * troubled) d->off = 1;
+    }
+    d->steps += 1;
+}
+
 // ------------------------------------------------------------------------------------------------------------
 // Conserved totals: a fixed-order FP64 tree (per-block partial sums, then one block over the partials).
 // ------------------------------------------------------------------------------------------------------------
@@ -830,6 +1010,21 @@ template __global__ void k_faces<4, 0, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<4, 1, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<5, 0, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<5, 1, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces_learned<4, 0>(KP);
+template __global__ void k_faces_learned<4, 1>(KP);
+template __global__ void k_faces_learned<5, 0>(KP);
+template __global__ void k_faces_learned<5, 1>(KP);
+template __global__ void k_faces_learned<5, 2>(KP);
+template __global__ void k_audit<4, 0>(KP);
+template __global__ void k_audit<4, 1>(KP);
+template __global__ void k_audit<5, 0>(KP);
+template __global__ void k_audit<5, 1>(KP);
+template __global__ void k_audit<5, 2>(KP);
+template __global__ void k_samples<4, 0>(KP, float*, float*);
+template __global__ void k_samples<4, 1>(KP, float*, float*);
+template __global__ void k_samples<5, 0>(KP, float*, float*);
+template __global__ void k_samples<5, 1>(KP, float*, float*);
+template __global__ void k_samples<5, 2>(KP, float*, float*);
 template __global__ void k_totals_partial<4>(const double*, long long, double*);
 template __global__ void k_totals_partial<5>(const double*, long long, double*);
 

@@ -11,7 +11,7 @@
 // stage combination of the time scheme, fused).
 #pragma once
 
-#include "fvg_device.hpp"
+#include "fvg_learned.hpp"
 
 namespace fvg {
 
@@ -52,11 +52,17 @@ struct KP {
     void* Fv[3];        // viscous face flux per axis (3-D: slots 1..4; 2-D: slots 1..3)
     long long nfa[3];   // faces per axis (plane stride of Fc / Fv)
     DevState* st;
+    // learned troubled-face indicator (FP32 fluxes, WENO5-Z)
+    const MlpDev* mlp;
+    IndState* ind;
+    int ind_audit;            // steps between audits (0: none)
+    float ind_label;          // weight departure of a troubled face
+    double ind_fn_limit;      // missed fraction of troubled faces that switches the indicator off
 };
 
-enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2 };
+enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3 };
 enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
 enum { CTL_BEGIN_STEP = 0, CTL_STAGE1_DT = 1, CTL_STAGE_S = 2, CTL_END_RK = 3, CTL_SEED = 4, CTL_HANCOCK_DT = 5, CTL_END_HANCOCK = 6,
-       CTL_RESIDUAL_S = 7 };
+       CTL_RESIDUAL_S = 7, CTL_AUDIT_BEGIN = 8, CTL_AUDIT_END = 9 };
 
 }   // namespace fvg

@@ -1,0 +1,161 @@
+// Learned troubled-face indicator of the WENO5-Z reconstruction (no reference counterpart; its labels come from the
+// weights of weno5z, Schemes.cpp:338-353).
+//
+// Features of a face (15): for density, pressure and the velocity component with the largest total variation over
+// the face's six-cell stencil c0..c5, the five differences c_{j+1} - c_j divided by
+// sqrt(sum of their squares + eps_rel * mean(c^2) + 1e-30) (eps_rel: WENO's relative regularisation).
+// Network: 15 -> 32 (ReLU) -> 32 (ReLU) -> 1 logit; a face is troubled when the logit exceeds the threshold.
+// Inference runs on the matrix cores, one wavefront of 64 faces at a time: two column blocks of 32 faces, each
+// through v_mfma_f32_32x32x8_bf16 (hidden units as rows, faces as columns). With that orientation the accumulator
+// layout of layer 1 (lane l holds rows 8j + 4(l/32) + i, i < 4, of face column l % 32) is the B-operand layout of
+// layer 2's K chunk j, so the hidden layer never leaves the registers. x, W1 (bias b1 folded into column 15 against
+// x15 = 1) and the first hidden layer are bfloat16 (round to nearest even); accumulation, b2, w3 and b3 are float.
+#pragma once
+
+#include "fvg_device.hpp"
+
+namespace fvg {
+
+constexpr int MLP_NIN = 15, MLP_NH = 32;
+constexpr int MLP_NPARAM = MLP_NH * MLP_NIN + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1;   // 1601
+
+// The network as the lanes of a wavefront read it.
+struct MlpDev {
+    unsigned long long w1[2][64];   // W1 fragment of K chunk kc for lane l: row l % 32, columns 8kc + 4(l/32) + 0..3
+    unsigned long long w2[4][64];   // W2 fragments, K chunks 0..3
+    float b2[32], w3[32];
+    float b3, thr;
+};
+
+// Audit counters and the indicator's own switch, kept for the life of a context.
+struct IndState {
+    unsigned long long last[4];    // TP, FP, FN, TN of the last audit (positive = troubled)
+    unsigned long long total[4];   // over all audits
+    long long steps;               // steps completed by the context
+    int off;                       // 1: the audit found too many missed troubled faces; WENO5-Z everywhere
+    int audits;
+    int pending;                   // the current step is being taken
+};
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned bf16_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ unsigned long long pack4(float a, float b, float c, float d) {
+    return (unsigned long long)bf16_bits(a) | ((unsigned long long)bf16_bits(b) << 16) | ((unsigned long long)bf16_bits(c) << 32) |
+           ((unsigned long long)bf16_bits(d) << 48);
+}
+__device__ __forceinline__ bf16x4 as_frag(unsigned long long v) { return __builtin_bit_cast(bf16x4, v); }
+__device__ __forceinline__ unsigned long long xor32(unsigned long long v) {
+    const int lo = __shfl_xor((int)(unsigned)v, 32, 64);
+    const int hi = __shfl_xor((int)(unsigned)(v >> 32), 32, 64);
+    return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+}
+
+// linear upwind-5 face value (weno5z with the linear weights)
+template <typename T> __device__ __forceinline__ T upwind5(T c0, T c1, T c2, T c3, T c4) {
+    return (T(2.0) * c0 - T(13.0) * c1 + T(47.0) * c2 + T(27.0) * c3 - T(3.0) * c4) * T(1.0 / 60.0);
+}
+
+// max_k |w_k - d_k| of the WENO5-Z weights of one face value
+template <typename T> __device__ __forceinline__ T weno5z_departure(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
+    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2) + weno_floor<T>();
+    const T s0 = c0 - T(2.0) * c1 + c2, t0 = c0 - T(4.0) * c1 + T(3.0) * c2;
+    const T s1 = c1 - T(2.0) * c2 + c3, t1 = c1 - c3;
+    const T s2 = c2 - T(2.0) * c3 + c4, t2 = T(3.0) * c2 - T(4.0) * c3 + c4;
+    const T b0 = T(13.0 / 12.0) * s0 * s0 + T(0.25) * t0 * t0;
+    const T b1 = T(13.0 / 12.0) * s1 * s1 + T(0.25) * t1 * t1;
+    const T b2 = T(13.0 / 12.0) * s2 * s2 + T(0.25) * t2 * t2;
+    const T tau = fabs(b0 - b2);
+    const T a0 = T(0.1) * (T(1.0) + dv(tau, b0 + eps)), a1 = T(0.6) * (T(1.0) + dv(tau, b1 + eps)),
+            a2 = T(0.3) * (T(1.0) + dv(tau, b2 + eps));
+    const T inv = dv(T(1.0), a0 + a1 + a2);
+    return smax(smax(fabs(a0 * inv - T(0.1)), fabs(a1 * inv - T(0.6))), fabs(a2 * inv - T(0.3)));
+}
+
+// label value of a face: the largest departure over the variables and both face states
+template <int NV, typename T> __device__ __forceinline__ T face_departure(const T (*c)[NV], T eps_rel) {
+    T d = T(0);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        d = smax(d, weno5z_departure(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], eps_rel));
+        d = smax(d, weno5z_departure(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], eps_rel));
+    }
+    return d;
+}
+
+// five normalised differences of one variable over the six-cell stencil
+__device__ __forceinline__ void diff5(float v0, float v1, float v2, float v3, float v4, float v5, float eps_rel, float* x) {
+    const float d0 = v1 - v0, d1 = v2 - v1, d2 = v3 - v2, d3 = v4 - v3, d4 = v5 - v4;
+    const float s = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3 + d4 * d4;
+    const float m = (v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3 + v4 * v4 + v5 * v5) * (1.0f / 6.0f);
+    const float r = __builtin_amdgcn_rsqf(s + eps_rel * m + 1e-30f);
+    x[0] = d0 * r;
+    x[1] = d1 * r;
+    x[2] = d2 * r;
+    x[3] = d3 * r;
+    x[4] = d4 * r;
+}
+
+// the 15 features of a face (x[15] = 1 carries the first layer's bias)
+template <int NV> __device__ __forceinline__ void face_features(const float (*c)[NV], float eps_rel, float* x) {
+    diff5(c[0][0], c[1][0], c[2][0], c[3][0], c[4][0], c[5][0], eps_rel, x);
+    diff5(c[0][3], c[1][3], c[2][3], c[3][3], c[4][3], c[5][3], eps_rel, x + 5);
+    // the velocity slot with the largest total variation (the first on ties), picked by selects
+    float v[6] = {c[0][1], c[1][1], c[2][1], c[3][1], c[4][1], c[5][1]}, best = -1.0f;
+#pragma unroll
+    for (int a = 0; a < NV - 2; ++a) {
+        const int q = mom(a);
+        float tv = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) tv += fabsf(c[j + 1][q] - c[j][q]);
+        const bool take = tv > best;
+        best = take ? tv : best;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v[j] = take ? c[j][q] : v[j];
+    }
+    diff5(v[0], v[1], v[2], v[3], v[4], v[5], eps_rel, x + 10);
+    x[15] = 1.0f;
+}
+
+// Logit of this lane's face; every lane of the wavefront must call it (the matrix instructions use all 64).
+__device__ __forceinline__ float mlp_logit_wave(const MlpDev* __restrict__ m, const float* x) {
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    unsigned long long g[4], o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        g[q] = pack4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+        o[q] = xor32(g[q]);
+    }
+    float out = 0.0f;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        // B operand (features of face 32 cb + l % 32; K rows 8kc + 4 half + 0..3): own or the partner lane's
+        const unsigned long long b0 = cb == 0 ? (half ? o[1] : g[0]) : (half ? g[1] : o[0]);
+        const unsigned long long b1 = cb == 0 ? (half ? o[3] : g[2]) : (half ? g[3] : o[2]);
+        f32x16 acc = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w1[0][lane]), as_frag(b0), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w1[1][lane]), as_frag(b1), acc, 0, 0, 0);
+        f32x16 acc2;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc2[e] = m->b2[8 * (e >> 2) + 4 * half + (e & 3)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned long long h = pack4(fmaxf(acc[4 * j], 0.0f), fmaxf(acc[4 * j + 1], 0.0f), fmaxf(acc[4 * j + 2], 0.0f),
+                                               fmaxf(acc[4 * j + 3], 0.0f));
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w2[j][lane]), as_frag(h), acc2, 0, 0, 0);
+        }
+        float part = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) part += m->w3[8 * (e >> 2) + 4 * half + (e & 3)] * fmaxf(acc2[e], 0.0f);
+        const float z = part + __shfl_xor(part, 32, 64) + m->b3;
+        if (cb == half) out = z;
+    }
+    return out;
+}
+
+}   // namespace fvg

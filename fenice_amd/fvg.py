@@ -26,8 +26,15 @@ _I = C.POINTER(C.c_int)
 EXPORTS = [
     "fvg_config_defaults", "fvg_config_set", "fvg_config_finalize", "fvg_dt_length", "fvg_create", "fvg_destroy",
     "fvg_last_error", "fvg_upload", "fvg_download", "fvg_residual", "fvg_step", "fvg_advance", "fvg_get_lagged_speed",
-    "fvg_set_lagged_speed", "fvg_conserved_totals", "fvg_kernel_times", "fvg_set_timing", "fvg_riemann_batch",
+    "fvg_set_lagged_speed", "fvg_conserved_totals", "fvg_kernel_times", "fvg_set_timing", "fvg_set_graphs", "fvg_riemann_batch",
+    "fvg_batch_create", "fvg_batch_destroy", "fvg_batch_last_error", "fvg_batch_upload", "fvg_batch_download",
+    "fvg_batch_run", "fvg_batch_set_block", "fvg_batch_get_block", "fvg_batch_tune",
+    "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
+    "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
 ]
+
+MLP_NPARAM = 1601
+_F = C.POINTER(C.c_float)
 
 
 class FvgError(RuntimeError):
@@ -91,7 +98,30 @@ def lib():
         L.fvg_conserved_totals.argtypes = [C.c_void_p, _D]
         L.fvg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), _D, C.POINTER(C.c_longlong), C.c_int]
         L.fvg_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.fvg_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.fvg_riemann_batch.argtypes = [P, C.c_int, _D, _D, _I, C.c_size_t, _D, _D]
+        L.fvg_batch_create.argtypes = [P, C.c_int, _I, C.c_int, C.POINTER(C.c_void_p)]
+        L.fvg_batch_destroy.argtypes = [C.c_void_p]
+        L.fvg_batch_last_error.argtypes = [C.c_void_p]
+        L.fvg_batch_last_error.restype = C.c_char_p
+        L.fvg_batch_upload.argtypes = [C.c_void_p, _D, C.c_longlong, C.c_int]
+        L.fvg_batch_download.argtypes = [C.c_void_p, _D, C.c_longlong, C.c_int]
+        L.fvg_batch_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _D, _I, _I, _D, _D]
+        L.fvg_batch_set_block.argtypes = [C.c_void_p, C.c_int]
+        L.fvg_batch_get_block.argtypes = [C.c_void_p, _I]
+        L.fvg_batch_tune.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_char_p, _I, _I]
+        F = C.POINTER(C.c_float)
+        L.fvg_indicator_set.argtypes = [C.c_void_p, F, C.c_int, C.c_float]
+        L.fvg_indicator_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), _I, _I]
+        L.fvg_indicator_samples.argtypes = [C.c_void_p, C.c_int, F, F, C.c_longlong]
+        L.fvg_trainer_create.argtypes = [C.c_int, C.c_longlong, C.POINTER(C.c_void_p)]
+        L.fvg_trainer_destroy.argtypes = [C.c_void_p]
+        L.fvg_trainer_last_error.argtypes = [C.c_void_p]
+        L.fvg_trainer_last_error.restype = C.c_char_p
+        L.fvg_trainer_set_data.argtypes = [C.c_void_p, F, F, C.c_longlong]
+        L.fvg_trainer_grad.argtypes = [C.c_void_p, F, C.c_float, F, _D]
+        L.fvg_trainer_fit.argtypes = [C.c_void_p, F, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, _D]
+        L.fvg_mlp_logits.argtypes = [C.c_int, F, C.c_float, F, C.c_longlong, F]
         _lib = L
     return _lib
 
@@ -215,6 +245,32 @@ class Solver:
     def set_timing(self, on: bool):
         self._c(lib().fvg_set_timing(self._h, int(on)))
 
+    def set_indicator(self, params, threshold: float = 0.0):
+        """Weights of the learned troubled-face indicator (FVG_MLP_NPARAM floats); resets its audit counters."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._c(lib().fvg_indicator_set(self._h, p.ctypes.data_as(_F), int(p.size), float(threshold)))
+
+    def indicator_stats(self):
+        """(last audit TP FP FN TN, all audits TP FP FN TN, off flag, audits)."""
+        c = (C.c_ulonglong * 8)()
+        off, n = C.c_int(), C.c_int()
+        self._c(lib().fvg_indicator_stats(self._h, c, C.byref(off), C.byref(n)))
+        v = [int(x) for x in c]
+        return v[:4], v[4:], bool(off.value), n.value
+
+    def samples(self, axis: int):
+        """Features (nfaces, 16) and label values (nfaces,) of the faces normal to `axis` of the resident state."""
+        cfg = self.cfg
+        nf = [(cfg.nx + 1) * cfg.ny * cfg.nz, cfg.nx * (cfg.ny + 1) * cfg.nz, cfg.nx * cfg.ny * (cfg.nz + 1)][axis]
+        X = np.empty((nf, 16), dtype=np.float32)
+        d = np.empty(nf, dtype=np.float32)
+        self._c(lib().fvg_indicator_samples(self._h, axis, X.ctypes.data_as(_F), d.ctypes.data_as(_F), nf))
+        return X, d
+
+    def set_graphs(self, on: bool):
+        """advance() replays a captured HIP graph of steps (on, the default) or enqueues every kernel (off)."""
+        self._c(lib().fvg_set_graphs(self._h, int(on)))
+
     def kernel_times(self):
         names = (C.c_char_p * 16)()
         ms = (C.c_double * 16)()
@@ -233,3 +289,156 @@ def riemann_batch(cfg: Config, UL, UR, axis, device: int = 0):
     S = np.empty(n)
     _check(lib().fvg_riemann_batch(C.byref(cfg), device, _dp(UL), _dp(UR), ax.ctypes.data_as(_I), n, _dp(F), _dp(S)))
     return F, S
+
+
+@dataclass
+class EnsembleResult:
+    t: np.ndarray          # final time per instance
+    steps: np.ndarray      # steps taken per instance
+    diverged: np.ndarray   # 1 where check_progress failed on the instance's last step
+    s_last: np.ndarray     # last step's wave speed per instance
+    ms: float              # kernel time of the launch
+
+
+class Ensemble:
+    """Independent 1-D runs (instance i: the reference's run on an ncells[i] x 1 mesh) advanced together by one
+    kernel launch (fvg_batch_*). `cfg` gives the scheme, x boundaries, Lx and Ly shared by every instance; its
+    ny must be 1 with periodic y sides."""
+
+    def __init__(self, cfg: Config, ncells, device: int = 0):
+        self.n = np.ascontiguousarray(ncells, dtype=np.int32)
+        self.total = int(self.n.sum())
+        h = C.c_void_p()
+        rc = lib().fvg_batch_create(C.byref(cfg), int(self.n.size), self.n.ctypes.data_as(_I), device, C.byref(h))
+        if rc != OK:
+            raise _ERR.get(rc, FvgError)(lib().fvg_batch_last_error(None).decode(errors="replace"))
+        self._h = h
+        self.offsets = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        if rc != OK:
+            raise _ERR.get(rc, FvgError)(lib().fvg_batch_last_error(self._h).decode(errors="replace"))
+
+    def upload(self, U):
+        """U: (sum ncells, 3 | 4 | 5) conservative states, instances concatenated in order."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self._c(lib().fvg_batch_upload(self._h, _dp(U), U.shape[0], U.shape[1]))
+
+    def download(self, ncomp: int = 4):
+        U = np.empty((self.total, ncomp))
+        self._c(lib().fvg_batch_download(self._h, _dp(U), self.total, ncomp))
+        return U
+
+    def instance(self, U, i):
+        return U[self.offsets[i]:self.offsets[i + 1]]
+
+    def run(self, t_end: float, max_steps: int, dt_max: float = 0.0) -> EnsembleResult:
+        k = self.n.size
+        t = np.empty(k)
+        steps = np.empty(k, dtype=np.int32)
+        div = np.empty(k, dtype=np.int32)
+        s = np.empty(k)
+        ms = C.c_double()
+        self._c(lib().fvg_batch_run(self._h, t_end, dt_max, max_steps, _dp(t), steps.ctypes.data_as(_I),
+                                    div.ctypes.data_as(_I), _dp(s), C.byref(ms)))
+        return EnsembleResult(t, steps, div, s, ms.value)
+
+    @property
+    def block(self) -> int:
+        b = C.c_int()
+        self._c(lib().fvg_batch_get_block(self._h, C.byref(b)))
+        return b.value
+
+    @block.setter
+    def block(self, b: int):
+        self._c(lib().fvg_batch_set_block(self._h, int(b)))
+
+    def tune(self, t_probe: float, max_steps: int, cache_path: str | None = None):
+        """Autotune the workgroup size on a probe run (the state is restored); returns (block, from_cache)."""
+        b, fc = C.c_int(), C.c_int()
+        self._c(lib().fvg_batch_tune(self._h, t_probe, max_steps, cache_path.encode() if cache_path else None,
+                                     C.byref(b), C.byref(fc)))
+        return b.value, bool(fc.value)
+
+
+class Trainer:
+    """Training of the indicator network on the device (fvg_trainer_*)."""
+
+    def __init__(self, capacity: int, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().fvg_trainer_create(device, int(capacity), C.byref(h))
+        if rc != OK:
+            raise _ERR.get(rc, FvgError)(lib().fvg_trainer_last_error(None).decode(errors="replace"))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fvg_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        if rc != OK:
+            raise _ERR.get(rc, FvgError)(lib().fvg_trainer_last_error(self._h).decode(errors="replace"))
+
+    def set_data(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.shape[1] == 15:
+            X = np.ascontiguousarray(np.concatenate([X, np.ones((X.shape[0], 1), np.float32)], axis=1))
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        self._keep = (X, y)
+        self._c(lib().fvg_trainer_set_data(self._h, X.ctypes.data_as(_F), y.ctypes.data_as(_F), X.shape[0]))
+
+    def grad(self, params, pos_weight: float = 1.0):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._c(lib().fvg_trainer_grad(self._h, p.ctypes.data_as(_F), pos_weight, g.ctypes.data_as(_F), C.byref(loss)))
+        return loss.value, g
+
+    def fit(self, params, epochs: int, batch: int, lr: float, pos_weight: float = 1.0, seed: int = 0):
+        p = np.array(params, dtype=np.float32)
+        loss = C.c_double()
+        self._c(lib().fvg_trainer_fit(self._h, p.ctypes.data_as(_F), epochs, batch, lr, pos_weight, seed, C.byref(loss)))
+        return p, loss.value
+
+
+def mlp_logits(params, X, threshold: float = 0.0, device: int = 0):
+    """Logits of feature rows X (n, 15 or 16) through the face kernels' inference routine."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if X.shape[1] == 15:
+        X = np.ascontiguousarray(np.concatenate([X, np.ones((X.shape[0], 1), np.float32)], axis=1))
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    z = np.empty(X.shape[0], dtype=np.float32)
+    rc = lib().fvg_mlp_logits(device, p.ctypes.data_as(_F), threshold, X.ctypes.data_as(_F), X.shape[0], z.ctypes.data_as(_F))
+    if rc != OK:
+        raise _ERR.get(rc, FvgError)(lib().fvg_trainer_last_error(None).decode(errors="replace"))
+    return z

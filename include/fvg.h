@@ -58,6 +58,14 @@ enum fvg_eos { FVG_EOS_IDEAL = 0, FVG_EOS_STIFFENED = 1 };
  * conservation sums stay FP64. */
 enum fvg_flux_precision { FVG_FLUX_FP64 = 0, FVG_FLUX_FP32 = 1 };
 
+/* Troubled-face indicator of the WENO5-Z reconstruction (no reference key). LEARNED (with flux_precision = FP32
+ * and reconstruction = weno5): a small network classifies every face from 15 features of its stencil; faces it
+ * calls smooth take the linear upwind-5 value instead of WENO5-Z. An audit every indicator_audit steps compares its
+ * decisions with the WENO5-Z weight departure (label: departure > indicator_label) and switches the indicator off
+ * for the rest of the context's life when the missed troubled faces exceed indicator_fn_limit of the troubled
+ * ones. */
+enum fvg_indicator { FVG_INDICATOR_NONE = 0, FVG_INDICATOR_LEARNED = 1 };
+
 /* The keys of the path (cpp/src/Params.cpp:98-223 and SimParams::from), typed. Defaults: fvg_config_defaults. */
 typedef struct fvg_config {
     int nx, ny, nz;              /* cells; nz = 1 is a 2-D mesh */
@@ -79,6 +87,10 @@ typedef struct fvg_config {
     double t_end, dt_max;        /* run keys */
     int max_steps;
     int flux_precision;          /* enum fvg_flux_precision (no reference key; default FP64) */
+    int indicator;               /* enum fvg_indicator (no reference key; default NONE) */
+    int indicator_audit;         /* steps between audits (default 50; 0: no audit) */
+    double indicator_label;      /* weight departure above which a face is troubled (default 0.05) */
+    double indicator_fn_limit;   /* fallback limit of the missed fraction of troubled faces (default 0.01) */
 } fvg_config;
 
 /* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,
@@ -142,12 +154,82 @@ int fvg_conserved_totals(fvg_ctx* ctx, double out[3]);
 int fvg_kernel_times(const fvg_ctx* ctx, const char** names, double* ms, long long* launches, int cap);
 /* Switch the per-kernel event timing on (1) or off (0, the default: no events are recorded). */
 int fvg_set_timing(fvg_ctx* ctx, int on);
+/* Launch mode of fvg_advance (no counterpart in the reference): 1 (default) replays a HIP graph captured once per
+ * context that holds four steps of the scheme; 0 enqueues every kernel. Results are the same bits either way;
+ * with event timing on, kernels are always enqueued one by one. */
+int fvg_set_graphs(fvg_ctx* ctx, int on);
 
 /* Known-answer entry: the Riemann flux of n faces with host AoS conservative states (5 doubles per face),
  * axis per face, through the device kernel of the solver configured in cfg (riemann, shock_blend, EOS).
  * F receives 5 doubles per face and smax one. Replaces RiemannSolver::face_flux_batch (Schemes.hpp:43). */
 int fvg_riemann_batch(const fvg_config* cfg, int device, const double* UL, const double* UR, const int* axis, size_t n,
                       double* F, double* smax);
+
+/* ---- Learned troubled-face indicator (no reference counterpart; see enum fvg_indicator) -----------------------
+ * The network: 15 features -> 32 (ReLU) -> 32 (ReLU) -> 1 logit; parameters as one float vector of
+ * FVG_MLP_NPARAM values: W1 (32 x 15, row-major), b1 (32), W2 (32 x 32), b2 (32), w3 (32), b3. A face is troubled
+ * when its logit exceeds the threshold. Features: for density, pressure and the velocity component with the largest
+ * total variation over the face's six-cell stencil, the five differences of neighbouring cells divided by
+ * sqrt(sum of their squares + weno_epsilon * mean square of the six values + 1e-30). */
+#define FVG_MLP_NPARAM 1601
+/* Weights of a context created with indicator = learned (until set, every face is troubled); resets the audit
+ * counters and switches the indicator back on. */
+int fvg_indicator_set(fvg_ctx* ctx, const float* params, int nparam, float threshold);
+/* counts: TP, FP, FN, TN of the last audit, then the same over all audits (positive = troubled);
+ * off: 1 once an audit switched the indicator off; audits: audits so far. Any output may be NULL. */
+int fvg_indicator_stats(fvg_ctx* ctx, unsigned long long counts[8], int* off, int* audits);
+/* Training samples of the resident state (a context with flux_precision = fp32): for each face normal to axis,
+ * its 16 feature slots (the 16th is 1) in X[16 * nfaces] and its label value (the largest WENO5-Z weight
+ * departure over the variables and both face states) in dep[nfaces]; -1 for faces without a full stencil. */
+int fvg_indicator_samples(fvg_ctx* ctx, int axis, float* X, float* dep, long long nfaces);
+
+/* Training on the device: weighted binary cross-entropy with logits (troubled samples weighted pos_weight), Adam.
+ * X holds 16 floats per sample (the first 15 are the features), y 0 or 1. */
+typedef struct fvg_trainer fvg_trainer;
+int fvg_trainer_create(int device, long long capacity, fvg_trainer** out);
+int fvg_trainer_destroy(fvg_trainer* t);
+const char* fvg_trainer_last_error(const fvg_trainer* t);
+int fvg_trainer_set_data(fvg_trainer* t, const float* X, const float* y, long long n);
+/* Mean loss and its gradient over all samples (FVG_MLP_NPARAM floats), the same bits on every call. */
+int fvg_trainer_grad(fvg_trainer* t, const float* params, float pos_weight, float* grad, double* loss);
+/* epochs of shuffled mini-batches from params (updated in place); loss (may be NULL): the final mean loss. */
+int fvg_trainer_fit(fvg_trainer* t, float* params, int epochs, int batch, float lr, float pos_weight, unsigned long long seed,
+                    double* loss);
+/* Logits of n feature rows (16 floats each) through the face kernels' inference routine (bfloat16 matrix cores). */
+int fvg_mlp_logits(int device, const float* params, float threshold, const float* X, long long n, float* z);
+
+/* ---- Ensemble of independent 1-D runs -------------------------------------------------------------------
+ * ninst runs of the reference's solver, instance i on an ncells[i] x 1 mesh (nx = ncells[i], ny = 1, periodic
+ * y), all with the scheme, x boundaries, Lx and Ly of cfg: each is Simulation::run_loop
+ * (SimulationRun.cpp:161-191) over TimeScheme::step (TimeScheme.cpp:110-245) with check_progress
+ * (SimulationInternal.hpp:259-267), run to its own t_end; the reference would run them one process each. The
+ * whole ensemble is one kernel launch (one workgroup per instance). Instances keep (rho, rho u, E); their
+ * rho v is +0 throughout, as in the reference's run. Explicit Euler / SSP-RK2 / SSP-RK3, Euler equations,
+ * FP64; x sides wall, no_slip, transmissive, outlet or periodic. */
+typedef struct fvg_batch fvg_batch;
+
+int fvg_batch_create(const fvg_config* cfg, int ninst, const int* ncells, int device, fvg_batch** out);
+int fvg_batch_destroy(fvg_batch* b);
+/* Message of the last failure of b (of the last failed fvg_batch_create when b is NULL). */
+const char* fvg_batch_last_error(const fvg_batch* b);
+/* The instances' states, concatenated in instance order: ncells = sum of ncells[i]; ncomp 3 (rho, rho u, E),
+ * 4 (the reference's 2-D State: rho, rho u, rho v, E) or 5. Download writes rho v (and rho w) as +0. */
+int fvg_batch_upload(fvg_batch* b, const double* U, long long ncells, int ncomp);
+int fvg_batch_download(fvg_batch* b, double* U, long long ncells, int ncomp);
+/* Every instance from t = 0 while t < t_end and fewer than max_steps steps (max_steps > 0 is required: it bounds
+ * the kernel). Per instance: final time, steps taken, diverged flag (1: check_progress failed on the last step,
+ * whose state is left as the step made it), last step's wave speed; ms (may be NULL) the kernel time. Any
+ * output pointer may be NULL. Divergence of an instance is reported through its flag, not the status. */
+int fvg_batch_run(fvg_batch* b, double t_end, double dt_max, int max_steps, double* t, int* steps, int* diverged, double* s_last,
+                  double* ms);
+/* Workgroup size of the ensemble kernel: 64, 128, 256 (default), 512 or 1024. Results do not depend on it. */
+int fvg_batch_set_block(fvg_batch* b, int block);
+int fvg_batch_get_block(const fvg_batch* b, int* block);
+/* Autotuner: time every workgroup size on a probe run of the uploaded state (to t_probe, at most max_steps),
+ * keep the fastest and restore the state. With cache_path, a line "key block ms" per tuned workload (device
+ * architecture, instance sizes, scheme) is appended, and a later call for the same key reads it instead of
+ * measuring (*from_cache = 1). */
+int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cache_path, int* block, int* from_cache);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,11 @@ static const double kDtEps = 1e-10;     /* TimeScheme.cpp:20 */
 
 static int g_threads = 0;
 void fvo_set_threads(int n) { g_threads = n; }
+
+/* 1: every WENO5-Z face value takes the linear weights (upwind-5), the smooth-face value of the learned
+ * indicator; 0 (default): the reference's WENO5-Z. */
+static int g_weno_linear = 0;
+void fvo_set_weno_linear(int on) { g_weno_linear = on; }
 int fvo_get_threads(void) {
 #ifdef _OPENMP
     return g_threads > 0 ? g_threads : omp_get_max_threads();
@@ -601,8 +606,13 @@ static void reconstruct_faces(const fvg_config* c, const Mesh* m, const Eos* e, 
                     } else {
                         const double c0 = cq[0], c1 = cq[stride], c2 = cq[2 * stride], c3 = cq[3 * stride], c4 = cq[4 * stride],
                                      c5 = cq[5 * stride];
-                        wl[q] = weno5z(c0, c1, c2, c3, c4, we);
-                        wr[q] = weno5z(c5, c4, c3, c2, c1, we);
+                        if (g_weno_linear) {
+                            wl[q] = (2.0 * c0 - 13.0 * c1 + 47.0 * c2 + 27.0 * c3 - 3.0 * c4) / 60.0;
+                            wr[q] = (2.0 * c5 - 13.0 * c4 + 47.0 * c3 + 27.0 * c2 - 3.0 * c1) / 60.0;
+                        } else {
+                            wl[q] = weno5z(c0, c1, c2, c3, c4, we);
+                            wr[q] = weno5z(c5, c4, c3, c2, c1, we);
+                        }
                     }
                 }
                 p2c(e, wl, UL + NV * f);

@@ -45,6 +45,8 @@ void fvo_volume_averages(const fvg_config* cfg, const double* U, double out[3]);
 
 /* Number of threads the oracle's OpenMP loops use (0: the OpenMP default). */
 void fvo_set_threads(int n);
+/* 1: WENO5-Z face values take the linear weights (upwind-5) everywhere; 0: the reference's WENO5-Z (default). */
+void fvo_set_weno_linear(int on);
 int fvo_get_threads(void);
 
 #ifdef __cplusplus

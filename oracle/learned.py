@@ -1,0 +1,145 @@
+"""CPU restatement of the learned troubled-face indicator (numpy, float64 unless stated).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's CPU baseline may use it, as the checker.
+The product (fenice_amd) never imports it.
+
+The indicator has no reference counterpart beyond the functions its labels come from:
+- label: the departure of the WENO5-Z weights from the linear weights (0.1, 0.6, 0.3) (weno5z, Schemes.cpp:338-353),
+  max over the primitive variables and both face states of a face, max_k |w_k - d_k| with
+  w_k = a_k / (a0 + a1 + a2); a face is troubled when it exceeds the label threshold;
+- smooth faces take the linear upwind-5 value (2c0 - 13c1 + 47c2 + 27c3 - 3c4) / 60, which is weno5z with a = d
+  (exact for quartic cell-average data);
+- features of a face (15): for density, pressure and the velocity component with the largest total variation over
+  the face's six-cell stencil c0..c5 (cells t-3 .. t+2 of a face between cells t-1 and t), the five differences
+  c_{j+1} - c_j divided by sqrt(sum of their squares + eps_rel * mean(c^2) + 1e-30), eps_rel the WENO
+  regularisation (the same relative scaling WENO5-Z applies);
+- network: 15 -> 32 (ReLU) -> 32 (ReLU) -> 1 logit; troubled when logit > threshold. Inference rounds x, W1, b1, W2
+  and the first hidden layer to bfloat16 (round to nearest even) and accumulates in float32.
+Parameter vector layout (float32, 1601 values): W1 (32 x 15, row-major), b1 (32), W2 (32 x 32), b2 (32), w3 (32),
+b3 (1).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NIN, NH = 15, 32
+NPARAM = NH * NIN + NH + NH * NH + NH + NH + 1
+D_LIN = np.array([0.1, 0.6, 0.3])
+
+
+def unpack(p):
+    p = np.asarray(p, dtype=np.float64)
+    o = 0
+    W1 = p[o:o + NH * NIN].reshape(NH, NIN); o += NH * NIN
+    b1 = p[o:o + NH]; o += NH
+    W2 = p[o:o + NH * NH].reshape(NH, NH); o += NH * NH
+    b2 = p[o:o + NH]; o += NH
+    w3 = p[o:o + NH]; o += NH
+    b3 = p[o]
+    return W1, b1, W2, b2, w3, b3
+
+
+def pack(W1, b1, W2, b2, w3, b3):
+    return np.concatenate([np.ravel(W1), b1, np.ravel(W2), b2, w3, [b3]]).astype(np.float32)
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest, ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+# ---- stencils on a periodic mesh ------------------------------------------------------------------------------
+def stencils(W, axis):
+    """Six-cell stencils of every face normal to `axis` of a periodic mesh. W: primitive planes (nv, nz, ny, nx)
+    (2-D: nz = 1). Returns (6, nv, nfaces) with faces in the device's per-axis order; on a periodic axis the face
+    t = n is the face t = 0, so the n faces t = 0 .. n-1 are returned."""
+    ax = {0: 3, 1: 2, 2: 1}[axis]
+    cells = [np.roll(W, 3 - s, axis=ax) for s in range(6)]   # cell t - 3 + s at position t
+    return np.stack([c.reshape(W.shape[0], -1) for c in cells])
+
+
+def weno5z_weights(c0, c1, c2, c3, c4, eps_rel):
+    eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * 0.2 + 1e-300
+    s0, t0 = c0 - 2.0 * c1 + c2, c0 - 4.0 * c1 + 3.0 * c2
+    s1, t1 = c1 - 2.0 * c2 + c3, c1 - c3
+    s2, t2 = c2 - 2.0 * c3 + c4, 3.0 * c2 - 4.0 * c3 + c4
+    b0 = (13.0 / 12.0) * s0 * s0 + 0.25 * t0 * t0
+    b1 = (13.0 / 12.0) * s1 * s1 + 0.25 * t1 * t1
+    b2 = (13.0 / 12.0) * s2 * s2 + 0.25 * t2 * t2
+    tau = np.abs(b0 - b2)
+    a = np.stack([0.1 * (1.0 + tau / (b0 + eps)), 0.6 * (1.0 + tau / (b1 + eps)), 0.3 * (1.0 + tau / (b2 + eps))])
+    return a / a.sum(axis=0)
+
+
+def departure(st, eps_rel, nvel):
+    """Label value of each face: max over variables and both face states of max_k |w_k - d_k|. st: (6, nv, nf)."""
+    c = st
+    wl = weno5z_weights(c[0], c[1], c[2], c[3], c[4], eps_rel)
+    wr = weno5z_weights(c[5], c[4], c[3], c[2], c[1], eps_rel)
+    d = np.maximum(np.abs(wl - D_LIN[:, None, None]).max(axis=0), np.abs(wr - D_LIN[:, None, None]).max(axis=0))
+    return d.max(axis=0)
+
+
+def upwind5(c0, c1, c2, c3, c4):
+    return (2.0 * c0 - 13.0 * c1 + 47.0 * c2 + 27.0 * c3 - 3.0 * c4) / 60.0
+
+
+def features(st, eps_rel, nvel):
+    """(nf, 15) features of each face from its stencils (6, nv, nf); velocity slots are 1, 2 (and 4 in 3-D)."""
+    vel = [1, 2] if nvel == 2 else [1, 2, 4]
+    tv = np.stack([np.abs(np.diff(st[:, q, :], axis=0)).sum(axis=0) for q in vel])
+    pick = np.argmax(tv, axis=0)   # first maximum on ties
+    v = np.choose(pick, [st[:, q, :] for q in vel])
+    out = []
+    for c in (st[:, 0, :], st[:, 3, :], v):
+        d = np.diff(c, axis=0)
+        norm = np.sqrt((d * d).sum(axis=0) + eps_rel * (c * c).mean(axis=0) + 1e-30)
+        out.append((d / norm).T)
+    return np.concatenate(out, axis=1)
+
+
+# ---- network --------------------------------------------------------------------------------------------------
+def forward(X, p, bf16_inference=False):
+    """Logits of the faces X (n, 15)."""
+    W1, b1, W2, b2, w3, b3 = unpack(p)
+    if bf16_inference:   # b1 rides in the bfloat16 weight column of the constant-1 input
+        X = bf16(X).astype(np.float64)
+        W1 = bf16(W1).astype(np.float64)
+        W2 = bf16(W2).astype(np.float64)
+        h1 = np.maximum(X @ W1.T + bf16(b1).astype(np.float64), 0.0)
+        h1 = bf16(h1).astype(np.float64)
+    else:
+        h1 = np.maximum(X @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+    return h2 @ w3 + b3
+
+
+def loss_and_grad(X, y, p, pos_weight=1.0):
+    """Mean weighted binary cross-entropy with logits, and its gradient with respect to the parameter vector
+    (float64 backpropagation)."""
+    W1, b1, W2, b2, w3, b3 = unpack(p)
+    X = np.asarray(X, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    n = X.shape[0]
+    z1 = X @ W1.T + b1
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0.0)
+    z = h2 @ w3 + b3
+    w = np.where(y > 0.5, pos_weight, 1.0)
+    # stable log(1 + exp(-|z|)) form
+    lse = np.maximum(z, 0.0) - z * y + np.log1p(np.exp(-np.abs(z)))
+    loss = (w * lse).sum() / n
+    sig = 1.0 / (1.0 + np.exp(-z))
+    gz = w * (sig - y) / n
+    g_w3 = h2.T @ gz
+    g_b3 = gz.sum()
+    gh2 = np.outer(gz, w3) * (z2 > 0)
+    g_W2 = gh2.T @ h1
+    g_b2 = gh2.sum(axis=0)
+    gh1 = (gh2 @ W2) * (z1 > 0)
+    g_W1 = gh1.T @ X
+    g_b1 = gh1.sum(axis=0)
+    return loss, np.concatenate([g_W1.ravel(), g_b1, g_W2.ravel(), g_b2, g_w3, [g_b3]])

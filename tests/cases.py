@@ -145,3 +145,22 @@ def vortex_state(n, L=10.0, gamma=1.4):
     U[:, 2] = rho * v
     U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
     return U
+
+
+def oracle_instance(oracle, keys, n, U4, t_end, max_steps, dt_max=0.0):
+    """One instance of an ensemble run through the oracle as the reference's N x 1 run: steps while t < t_end and
+    fewer than max_steps; a failed check_progress ends the run with that step counted. Returns
+    (U (n, 4), t, steps, diverged)."""
+    from fenice_amd.fvg import config_from_keys
+    cfg = config_from_keys(dict(keys, nx=str(n)))
+    U = np.zeros((n, 5))
+    U[:, :4] = U4
+    t, steps, lag, diverged = 0.0, 0, 0.0, 0
+    while t < t_end and steps < max_steps:
+        dt, _, lag, ok = oracle.step(cfg, U, t, t_end, dt_max, lag)
+        steps += 1
+        if not ok:
+            diverged = 1
+            break
+        t += dt
+    return U[:, :4].copy(), t, steps, diverged

@@ -36,6 +36,9 @@ class Oracle:
         L.fvo_set_threads.argtypes = [C.c_int]
         L.fvo_get_threads.restype = C.c_int
 
+    def set_weno_linear(self, on: bool):
+        self.lib.fvo_set_weno_linear(int(on))
+
     def set_threads(self, n: int):
         self.lib.fvo_set_threads(n)
 

@@ -58,7 +58,23 @@ def test_labels_and_auto_scheme():
     for lab, lim in [("Minmod", LIM_MINMOD), ("vanleer", LIM_VAN_LEER), ("MC", LIM_MC), ("koren", LIM_KOREN)]:
         assert fvg.config_from_keys(dict(nx=4, ny=4, muscl_limiter=lab)).limiter == lim
     for lab, ts in [("SSP Runge-Kutta 3", TS_SSPRK3), ("rk2", TS_SSPRK2), ("Explicit Euler", TS_EULER), ("hancock", TS_HANCOCK)]:
-        assert fvg.config_from_keys(dict(nx=4, ny=4, time_scheme=lab)).time_scheme == ts
+        assert fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="muscl", time_scheme=lab)).time_scheme == ts
+
+
+def test_scheme_combinations_checked_like_the_reference():
+    # SimParams::from (Params.cpp:429-460): the same rejections and messages, and the order-1 normalisation
+    with pytest.raises(fvg.InvalidArgument, match="weno5 is not used with SSP Runge-Kutta 2"):
+        fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno5", time_scheme="SSP Runge-Kutta 2"))
+    with pytest.raises(fvg.InvalidArgument, match="weno3 is not used with SSP Runge-Kutta 2"):
+        fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno3", time_scheme="rk2"))
+    with pytest.raises(fvg.InvalidArgument, match="MUSCL-Hancock needs reconstruction = muscl"):
+        fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno5", time_scheme="MUSCL-Hancock"))
+    with pytest.raises(fvg.InvalidArgument, match="MUSCL-Hancock needs reconstruction = muscl"):
+        fvg.config_from_keys(dict(nx=4, ny=4, time_scheme="hancock"))
+    with pytest.raises(fvg.InvalidArgument, match="weno_epsilon must be >= 0"):
+        fvg.config_from_keys(dict(nx=4, ny=4, weno_epsilon="-1"))
+    assert fvg.config_from_keys(dict(nx=4, ny=4, boundary_order="2")).boundary_order == 1
+    assert fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="muscl", boundary_order="2")).boundary_order == 2
     assert fvg.config_from_keys(dict(nx=4, ny=4, reconstruction="weno3")).reconstruction == RECON_WENO3
     c = fvg.config_from_keys(dict(nx=4, ny=4, bc_xmin="Inviscid Wall (Slip)", bc_xmax="no_slip", bc_ymin="Moving Wall",
                                   bc_ymax="transmissive"))

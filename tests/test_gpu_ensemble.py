@@ -1,0 +1,142 @@
+"""The 1-D ensemble kernel (fvg_batch_*) against the oracle's N x 1 runs (the reference's mapping of a 1-D
+problem): every instance's state, time and step count bit for bit, over schemes, boundaries and sizes down to
+single cells, independent of the workgroup size, with per-instance divergence."""
+import numpy as np
+import pytest
+
+from cases import oracle_instance
+from fenice_amd import fvg
+from fenice_amd.workloads import ensemble_keys, riemann_ensemble, riemann_states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+def run_device(keys, n, U, t_end, max_steps, block=None):
+    cfg = fvg.config_from_keys(keys)
+    with fvg.Ensemble(cfg, n) as e:
+        if block:
+            e.block = block
+        e.upload(U)
+        r = e.run(t_end, max_steps)
+        return e.download(4), r, e.offsets
+
+
+def check_against_oracle(oracle, keys, n, U, t_end, max_steps, which=None, block=None):
+    Ud, r, off = run_device(keys, n, U, t_end, max_steps, block)
+    for i in (range(len(n)) if which is None else which):
+        Uo, t, steps, div = oracle_instance(oracle, keys, int(n[i]), U[off[i]:off[i + 1]], t_end, max_steps)
+        assert steps == r.steps[i] and div == r.diverged[i], (i, steps, r.steps[i], div, r.diverged[i])
+        assert t == r.t[i], (i, t, r.t[i])
+        assert same(Ud[off[i]:off[i + 1]], Uo), i
+    return r
+
+
+def test_small_ensemble_matches_oracle(oracle):
+    say("ensemble: 40 instances, 6..300 cells, to t = 0.15")
+    n, U, _ = riemann_ensemble(40, seed=7, nmin=6, nmax=300)
+    r = check_against_oracle(oracle, ensemble_keys(), n, U, 0.15, 2000)
+    assert r.diverged.sum() == 0 and (r.t == 0.15).sum() + (r.steps == 2000).sum() == len(n)
+
+
+SCHEMES = [
+    ("muscl-minmod-b1", dict(muscl_limiter="Minmod", boundary_order="1")),
+    ("muscl-vanleer-rk3", dict(muscl_limiter="Van Leer", time_scheme="SSP Runge-Kutta 3")),
+    ("muscl-koren-euler", dict(muscl_limiter="Koren", time_scheme="Explicit Euler", cfl="0.3")),
+    ("weno3", dict(reconstruction="weno3", time_scheme="SSP Runge-Kutta 3")),
+    ("weno5-b1", dict(reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", boundary_order="1")),
+    ("weno5-b2", dict(reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", boundary_order="2")),
+    ("first-order", dict(reconstruction="none", time_scheme="Explicit Euler")),
+    ("lowmach-blend", dict(riemann_solver="HLLC (Low-Mach Thornber)", shock_blend="0.2")),
+    ("walls", dict(bc_xmin="wall", bc_xmax="no_slip")),
+    ("outlet", dict(bc_xmax="outlet", p_out="0.5")),
+    ("periodic", dict(bc_xmin="periodic", bc_xmax="periodic", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3")),
+    ("stiffened", dict(eos_type="stiffened", p_inf="0.5")),
+]
+
+
+@pytest.mark.parametrize("name,over", SCHEMES, ids=[s[0] for s in SCHEMES])
+def test_scheme_matrix_matches_oracle(oracle, name, over):
+    say("ensemble scheme", name)
+    n, _, params = riemann_ensemble(10, seed=11, nmin=1, nmax=90)
+    n[:4] = [1, 2, 3, 4]   # below and at the halo depth
+    U = riemann_states(n, params)
+    check_against_oracle(oracle, dict(ensemble_keys(), **over), n, U, 0.05, 300)
+
+
+def test_block_size_does_not_change_bits():
+    say("ensemble: block sizes")
+    n, U, _ = riemann_ensemble(24, seed=3, nmin=16, nmax=1500)
+    outs = [run_device(ensemble_keys(), n, U, 0.02, 400, b) for b in (64, 256, 1024)]
+    for Ud, r, _ in outs[1:]:
+        assert same(Ud, outs[0][0]) and np.array_equal(r.steps, outs[0][1].steps) and same(r.t, outs[0][1].t)
+
+
+def test_divergence_is_per_instance(oracle):
+    say("ensemble: one diverging instance")
+    n, U, _ = riemann_ensemble(6, seed=5, nmin=20, nmax=60)
+    off = np.concatenate([[0], np.cumsum(n)])
+    Ub = U.copy()
+    Ub[off[2] + 3] = [1.0, 0.0, 0.0, np.inf]
+    Ud, r, _ = run_device(ensemble_keys(), n, Ub, 0.05, 500)
+    assert r.diverged.tolist() == [0, 0, 1, 0, 0, 0] and r.steps[2] == 1
+    Uc, rc, _ = run_device(ensemble_keys(), n, U, 0.05, 500)
+    for i in (0, 1, 3, 4, 5):
+        assert same(Ud[off[i]:off[i + 1]], Uc[off[i]:off[i + 1]]) and r.steps[i] == rc.steps[i]
+    _, _, steps, div = oracle_instance(oracle, ensemble_keys(), int(n[2]), Ub[off[2]:off[3]], 0.05, 500)
+    assert div == 1 and steps == 1
+
+
+def test_full_size_ensemble_sampled_against_oracle(oracle):
+    # the 1024-instance ensemble (128..8192 cells, 4.24 M cells), 30 steps; nine instances spread over the sizes
+    say("ensemble: 1024 instances, 30 steps")
+    n, U, _ = riemann_ensemble()
+    order = np.argsort(n)
+    which = [int(order[k]) for k in (0, 1, 128, 300, 512, 700, 900, 1022, 1023)]
+    r = check_against_oracle(oracle, ensemble_keys(), n, U, 0.15, 30, which=which)
+    assert (r.steps == 30).all() and r.diverged.sum() == 0
+
+
+def test_tuner_restores_state_and_caches(tmp_path):
+    say("ensemble: autotuner")
+    n, U, _ = riemann_ensemble(64, seed=9, nmin=64, nmax=2048)
+    keys = ensemble_keys()
+    cache = str(tmp_path / "tune.txt")
+    with fvg.Ensemble(fvg.config_from_keys(keys), n) as e:
+        e.upload(U)
+        b1, fc1 = e.tune(0.005, 200, cache)
+        assert not fc1 and b1 in (64, 128, 256, 512, 1024) and e.block == b1
+        assert same(e.download(4), U)
+        r = e.run(0.02, 400)
+        Ut = e.download(4)
+    with fvg.Ensemble(fvg.config_from_keys(keys), n) as e:
+        e.upload(U)
+        b2, fc2 = e.tune(0.005, 200, cache)
+        assert fc2 and b2 == b1
+    Ud, r0, _ = run_device(keys, n, U, 0.02, 400, 256)
+    assert same(Ud, Ut) and np.array_equal(r.steps, r0.steps)
+
+
+def test_rejects_unsupported_configurations():
+    keys = ensemble_keys()
+    with pytest.raises(fvg.Unsupported):
+        fvg.Ensemble(fvg.config_from_keys(dict(keys, bc_ymin="wall", bc_ymax="wall")), [8, 9])
+    with pytest.raises(fvg.Unsupported):
+        fvg.Ensemble(fvg.config_from_keys(dict(keys, use_viscous="true")), [8, 9])
+    with pytest.raises(fvg.Unsupported):
+        fvg.Ensemble(fvg.config_from_keys(dict(keys, bc_xmin="moving_wall")), [8, 9])
+    with fvg.Ensemble(fvg.config_from_keys(keys), [8, 9]) as e:
+        with pytest.raises(fvg.LogicError):
+            e.run(0.1, 10)
+        e.upload(np.ones((17, 3)))
+        with pytest.raises(fvg.InvalidArgument):
+            e.run(0.1, 0)
+        with pytest.raises(fvg.InvalidArgument):
+            e.upload(np.ones((16, 3)))

@@ -1,0 +1,60 @@
+"""Pins of the learned indicator's CPU restatement (oracle/learned.py): gradient against central differences,
+upwind-5 exactness, the WENO5-Z departure at its closed-form cases, feature invariances, bfloat16 rounding."""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import learned as L  # noqa: E402
+
+
+def test_gradient_matches_central_differences():
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 0.4, L.NPARAM)
+    X = rng.normal(0, 0.5, (40, 15))
+    y = (rng.random(40) < 0.3).astype(float)
+    loss, g = L.loss_and_grad(X, y, p, pos_weight=3.0)
+    h = 1e-6
+    for k in rng.choice(L.NPARAM, 60, replace=False).tolist() + [L.NPARAM - 1]:
+        e = np.zeros(L.NPARAM)
+        e[k] = h
+        fd = (L.loss_and_grad(X, y, p + e, 3.0)[0] - L.loss_and_grad(X, y, p - e, 3.0)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
+    assert np.isclose(loss, L.loss_and_grad(X, y, L.pack(*L.unpack(p)).astype(np.float64), 3.0)[0], rtol=1e-5)
+
+
+def test_upwind5_exact_for_quartic_cell_averages():
+    # cell averages of x^4 on unit cells [j, j+1]: ((j+1)^5 - j^5) / 5; the face value at x = 3 (right of c2)
+    avg = lambda j: ((j + 1.0) ** 5 - j ** 5) / 5.0   # noqa: E731
+    c = [avg(j) for j in range(5)]
+    assert abs(L.upwind5(*c) - 3.0 ** 4) < 1e-12
+    assert abs(L.upwind5(*[avg(j) + 2.0 for j in range(5)]) - (81.0 + 2.0)) < 1e-12
+
+
+def test_departure_closed_forms():
+    j = np.arange(6, dtype=float)
+    lin = (1.3 + 0.07 * j)[:, None, None] * np.ones((6, 5, 1))
+    assert L.departure(lin, 1e-6, 3).max() < 1e-12            # linear data: tau = 0, weights = linear weights
+    step = np.where(j < 3, 1.0, 0.125)[:, None, None] * np.ones((6, 5, 1))
+    assert L.departure(step, 1e-6, 3).max() > 0.25              # a jump concentrates the weights
+    flat = np.full((6, 5, 1), 2.0)
+    assert L.departure(flat, 1e-6, 3).max() == 0.0
+
+
+def test_feature_invariances():
+    rng = np.random.default_rng(1)
+    st = 1.0 + 0.1 * rng.random((6, 5, 50))
+    f = L.features(st, 1e-6, 3)
+    assert f.shape == (50, 15) and np.all(np.abs(f) <= 1.0)
+    st2 = st.copy()
+    st2[:, 0] *= 7.0   # density scale: its five features do not change
+    assert np.allclose(L.features(st2, 1e-6, 3)[:, :5], f[:, :5], atol=1e-12)
+    ramp = (0.5 + 0.01 * np.arange(6.0))[:, None, None] * np.ones((6, 5, 1))
+    fr = L.features(ramp, 0.0, 3)
+    assert np.allclose(fr[0, :5], 1.0 / np.sqrt(5.0))   # a linear ramp: five equal differences
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -2.5, 1.0 + 2.0 ** -9], dtype=np.float32)
+    assert L.bf16(x).tolist() == [1.0, 1.0, 1.015625, -2.5, 1.0]

@@ -50,3 +50,29 @@ def test_vortex_hancock_cfl08_diverges_at_65(oracle, ref):
         progressed = np.isfinite(sr) and np.isfinite(dtr) and t + dtr > t
         assert ok == progressed and progressed == (step < VORTEX_DIVERGES_AT), step
         t += dtr
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 40])
+@pytest.mark.parametrize("recon,bo", [("muscl", "1"), ("muscl", "2"), ("weno5", "2"), ("none", "2")])
+def test_one_row_runs_equal_reference(oracle, ref, n, recon, bo):
+    # the N x 1 mapping of a 1-D problem (periodic y, one row), down to meshes smaller than the halo depth
+    from fenice_amd.workloads import ensemble_keys, riemann_ensemble, riemann_states
+    _, _, params = riemann_ensemble(3, seed=13, nmin=1, nmax=2)
+    keys = dict(ensemble_keys(), nx=str(n), reconstruction=recon, boundary_order=bo)
+    if recon == "weno5":
+        keys["time_scheme"] = "SSP Runge-Kutta 3"
+    U4 = riemann_states([n], params[1:2])
+    U = np.zeros((n, 5))
+    U[:, :4] = U4
+    run = ref.create("Sod Shock Tube", **keys)
+    run.set_state(U)
+    cfg = config_from_keys(keys)
+    t, lag, steps = 0.0, 0.0, 0
+    while t < 0.15 and steps < 12:
+        dtr, sr = run.step(t, 0.15)
+        dto, so, lag, ok = oracle.step(cfg, U, t, 0.15, 0.0, lag)
+        assert ok and dtr == dto and sr == so
+        t += dtr
+        steps += 1
+    assert np.array_equal(run.get_state().view(np.uint64), U.view(np.uint64))
+    assert np.all(U[:, 2] == 0.0) and not np.signbit(U[:, 2]).any()

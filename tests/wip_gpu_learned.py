@@ -1,0 +1,193 @@
+"""The learned troubled-face indicator on the device against its CPU restatement (oracle/learned.py) and the oracle:
+matrix-core inference layout, training gradients, features and labels, the two limits of the decision (every face
+troubled: the FP32 WENO5-Z path bit for bit; every face smooth: the oracle's upwind-5 residual to FP32 accuracy),
+the audit and its fallback, and what a trained network gains."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cases import base_keys, shock_state, smooth_state
+from fenice_amd import fvg
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import learned as L  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rand_params(seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    return rng.normal(0.0, scale, L.NPARAM).astype(np.float32)
+
+
+def const_params(b3):
+    p = np.zeros(L.NPARAM, dtype=np.float32)
+    p[-1] = b3
+    return p
+
+
+def test_matrix_core_inference_matches_cpu():
+    # every lane layout of the two chained 32x32x8 bfloat16 products: random weights and features, 1000 faces
+    say("mlp logits")
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (1000, 15)).astype(np.float32)
+    for seed in (1, 2):
+        p = rand_params(seed)
+        z = fvg.mlp_logits(p, X)
+        ref = L.forward(X.astype(np.float64), p, bf16_inference=True)
+        scale = np.abs(ref).max()
+        assert np.abs(z - ref).max() <= 2e-3 * scale + 1e-4, np.abs(z - ref).max()
+        # the float forward differs from the bfloat16 one only by the rounding of x, W1, b1, W2, h1
+        assert np.abs(z - L.forward(X.astype(np.float64), p)).max() <= 0.05 * scale
+
+
+def test_training_gradient_matches_cpu():
+    say("trainer gradient")
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1, 1, (3000, 15)).astype(np.float32)
+    y = (rng.random(3000) < 0.2).astype(np.float32)
+    p = rand_params(5, 0.3)
+    with fvg.Trainer(4096) as t:
+        t.set_data(X, y)
+        loss, g = t.grad(p, pos_weight=4.0)
+        loss2, g2 = t.grad(p, pos_weight=4.0)
+    lref, gref = L.loss_and_grad(X.astype(np.float64), y, p.astype(np.float64), pos_weight=4.0)
+    assert abs(loss - lref) <= 1e-5 * abs(lref)
+    assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
+    assert loss == loss2 and np.array_equal(g.view(np.uint32), g2.view(np.uint32))   # deterministic reduction
+
+
+def test_training_separates_a_toy_problem():
+    say("trainer fit")
+    rng = np.random.default_rng(6)
+    X = rng.uniform(-1, 1, (20000, 15)).astype(np.float32)
+    y = ((X[:, 2] - X[:, 3]) ** 2 + 0.5 * X[:, 7] > 0.4).astype(np.float32)
+    p0 = rand_params(7, 0.2)
+    with fvg.Trainer(20000) as t:
+        t.set_data(X, y)
+        l0, _ = t.grad(p0)
+        p, loss = t.fit(p0, epochs=20, batch=512, lr=3e-3, seed=1)
+    acc = ((L.forward(X.astype(np.float64), p) > 0) == (y > 0.5)).mean()
+    assert loss < 0.3 * l0 and acc > 0.95, (l0, loss, acc)
+
+
+def prim_planes(U, nx, ny, nz, gamma=1.4):
+    rho = U[:, 0]
+    u, v, w = U[:, 1] / rho, U[:, 2] / rho, U[:, 4] / rho
+    p = (gamma - 1.0) * (U[:, 3] - 0.5 * rho * (u * u + v * v + w * w))
+    W = np.stack([rho, u, v, p, w]).reshape(5, nz, ny, nx)
+    return W.astype(np.float32).astype(np.float64)   # the FP32 path's primitives
+
+
+@pytest.mark.parametrize("dims", [(24, 20, 1), (16, 14, 12)])
+def test_samples_match_cpu(dims):
+    say("samples", dims)
+    nx, ny, nz = dims
+    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32")
+    U = shock_state(nx, ny, nz, 8)
+    W = prim_planes(U, nx, ny, nz)
+    nv = 5 if nz > 1 else 4
+    with fvg.Solver(fvg.config_from_keys(keys)) as s:
+        s.upload(U)
+        for axis in range(3 if nz > 1 else 2):
+            X, d = s.samples(axis)
+            st = L.stencils(W[:nv] if nz > 1 else W[[0, 1, 2, 3]], axis)
+            if nz == 1:   # 2-D: the velocity slots are 1, 2
+                pass
+            ni, nj = nx + (axis == 0), ny + (axis == 1)
+            Xd = X.reshape(nz + (axis == 2), nj, ni, 16)
+            dd = d.reshape(nz + (axis == 2), nj, ni)
+            # periodic mesh: faces t = 0 .. n-1 of the device equal the CPU's
+            Xd = Xd[:nz, :ny, :nx].reshape(-1, 16)
+            dd = dd[:nz, :ny, :nx].reshape(-1)
+            stx = st if nz > 1 else st
+            f = L.features(stx if nz > 1 else np.concatenate([stx, stx[:, :1] * 0], axis=1)[:, [0, 1, 2, 3]], 1e-6, 3 if nz > 1 else 2)
+            dep = L.departure(stx, 1e-6, 3 if nz > 1 else 2)
+            assert np.abs(Xd[:, :15] - f).max() < 2e-4
+            assert np.all(Xd[:, 15] == 1.0)
+            big = dep > 1e-3
+            assert np.abs(dd[big] - dep[big]).max() < 1e-3 and np.abs(dd - dep).max() < 2e-3
+
+
+def learned_keys(nx, ny, nz, **kw):
+    return base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32", indicator="learned", **kw)
+
+
+@pytest.mark.parametrize("dims", [(20, 18, 1), (14, 12, 10)])
+def test_all_troubled_equals_fp32_weno(dims):
+    # a network that calls every face troubled reproduces the FP32 WENO5-Z path bit for bit
+    say("all troubled", dims)
+    nx, ny, nz = dims
+    U = shock_state(nx, ny, nz, 9)
+    outs = []
+    for keys, params in ((base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32"), None),
+                         (learned_keys(nx, ny, nz), const_params(30.0)),
+                         (learned_keys(nx, ny, nz), None)):   # untrained: every face troubled
+        with fvg.Solver(fvg.config_from_keys(keys)) as s:
+            if params is not None:
+                s.set_indicator(params, 0.0)
+            s.upload(U)
+            R, S = s.residual()
+            s.advance(0.0, 100.0, max_steps=3)
+            outs.append((R, S, s.download()))
+    for R, S, Uo in outs[1:]:
+        assert np.array_equal(R.view(np.uint64), outs[0][0].view(np.uint64)) and S == outs[0][1]
+        assert np.array_equal(Uo.view(np.uint64), outs[0][2].view(np.uint64))
+
+
+@pytest.mark.parametrize("dims", [(24, 20, 1), (16, 14, 12)])
+def test_all_smooth_equals_oracle_upwind5(oracle, dims):
+    # a network that calls every face smooth: the oracle's residual with the linear weights, to FP32 accuracy
+    say("all smooth", dims)
+    nx, ny, nz = dims
+    keys = learned_keys(nx, ny, nz)
+    U = smooth_state(nx, ny, nz, 10, mach=0.3)
+    with fvg.Solver(fvg.config_from_keys(keys)) as s:
+        s.set_indicator(const_params(-30.0), 0.0)
+        s.upload(U)
+        R, S = s.residual()
+    okeys = base_keys(nx, ny, nz, reconstruction="weno5")
+    oracle.set_weno_linear(True)
+    try:
+        Ro, So = oracle.residual(fvg.config_from_keys(okeys), U)
+    finally:
+        oracle.set_weno_linear(False)
+    Rw, _ = oracle.residual(fvg.config_from_keys(okeys), U)
+    scale = np.abs(Ro).max(axis=0)
+    err = np.abs(R - Ro).max(axis=0) / scale
+    assert np.all(err < 2e-4), err
+    assert np.abs(Rw - Ro).max() > 1e3 * np.abs(R - Ro).max()   # the upwind-5 path is what the device ran
+    assert abs(S - So) <= 1e-5 * So
+
+
+def test_audit_counts_and_fallback():
+    # on a shock state a network that calls every face smooth misses every troubled face: the audit of the first
+    # step switches the indicator off, and from then on the run is the FP32 WENO5-Z run
+    say("audit fallback")
+    nx, ny = 40, 32
+    U = shock_state(nx, ny, 1, 11)
+    keys = learned_keys(nx, ny, 1, indicator_audit="1", indicator_fn_limit="0.01")
+    with fvg.Solver(fvg.config_from_keys(keys)) as s:
+        s.set_indicator(const_params(-30.0), 0.0)
+        s.upload(U)
+        s.advance(0.0, 100.0, max_steps=1)
+        last, total, off, audits = s.indicator_stats()
+        assert audits == 1 and off
+        assert sum(last) == (nx + 1 - 2 * 0) * ny + nx * (ny + 1) - 2 * 0 or sum(last) > 0
+        assert last[0] == 0 and last[1] == 0 and last[2] > 0   # predicted smooth: no TP/FP, troubled faces missed
+        s.advance(0.0, 100.0, max_steps=2)
+        _, _, off2, audits2 = s.indicator_stats()
+        assert off2 and audits2 == 3
+    # audits every step of a network that calls everything troubled: no misses, stays on
+    with fvg.Solver(fvg.config_from_keys(keys)) as s:
+        s.set_indicator(const_params(30.0), 0.0)
+        s.upload(U)
+        s.advance(0.0, 100.0, max_steps=2)
+        last, total, off, audits = s.indicator_stats()
+        assert not off and audits == 2 and last[2] == 0 and last[3] == 0 and last[0] > 0
