@@ -28,46 +28,38 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, published (FMA counted as 2)
 BYTES_PER_CELL_STAGE_RK3_3D = 8.0 * 5 * (2 + 3 + 3) / 3.0   # 106.7 B: stage input, U^n on stages 2-3, output
 
-
-def tgv_state(n: int, gamma: float = 1.4, mach: float = 0.1) -> np.ndarray:
-    h = 2.0 * math.pi / n
-    x = (np.arange(n) + 0.5) * h
-    z3, y3, x3 = np.meshgrid(x, x, x, indexing="ij")
-    u = np.sin(x3) * np.cos(y3) * np.cos(z3)
-    v = -np.cos(x3) * np.sin(y3) * np.cos(z3)
-    p = 1.0 / (gamma * mach * mach) + (np.cos(2 * x3) + np.cos(2 * y3)) * (np.cos(2 * z3) + 2.0) / 16.0
-    U = np.empty((n ** 3, 5))
-    U[:, 0] = 1.0
-    U[:, 1] = u.ravel()
-    U[:, 2] = v.ravel()
-    U[:, 3] = p.ravel() / (gamma - 1.0) + 0.5 * (u * u + v * v).ravel()
-    U[:, 4] = 0.0
-    return U
+from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
+from fenice_amd.workloads import taylor_green_keys as run_keys  # noqa: E402
 
 
-def run_keys(n: int, flux: str = "fp64") -> dict:
-    L = "6.283185307179586"
-    return dict(nx=n, ny=n, nz=n, Lx=L, Ly=L, Lz=L, gamma=1.4, use_viscous="true", mu="6.25e-4", pr="0.71",
-                bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", bc_zmin="periodic",
-                bc_zmax="periodic", riemann_solver="HLLC", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5",
-                flux_precision=flux)
-
-
-def error_vs_fp64(n: int, flux: str, steps: int = 20) -> dict:
-    """L1 / Linf relative error of the density of a `flux` run against the FP64 path (bit-identical to the oracle), both
-    from the Taylor-Green state at the same fixed dt (below the CFL step), after `steps` steps."""
+def make_solver(n: int, flux: str, indicator: str = "none", weights: str = ""):
+    """The solver of the workload; with indicator = learned, the trained network's weights are loaded."""
     from fenice_amd import fvg
+    keys = dict(run_keys(n, flux))
+    if indicator == "learned":
+        keys.update(indicator="learned", indicator_audit="50")
+    s = fvg.Solver(fvg.config_from_keys(keys))
+    if indicator == "learned":
+        from fenice_amd.train_indicator import DEFAULT_OUT, load
+        params, thr, _, _ = load(weights or DEFAULT_OUT)
+        s.set_indicator(params, thr)
+    return s
+
+
+def error_vs_fp64(n: int, flux: str, indicator: str = "none", weights: str = "", steps: int = 20) -> dict:
+    """L1 / Linf relative error of the density of a `flux` (and `indicator`) run against the FP64 path (bit-identical
+    to the oracle), both from the Taylor-Green state at the same fixed dt (below the CFL step), after `steps` steps."""
     U0 = tgv_state(n)
-    out = {}
-    for f in ("fp64", flux):
-        with fvg.Solver(fvg.config_from_keys(run_keys(n, f))) as s:
+    out = []
+    for f, ind in (("fp64", "none"), (flux, indicator)):
+        with make_solver(n, f, ind, weights) as s:
             s.upload(U0)
             t = 0.0
             for _ in range(steps):
                 t += s.step(t, 1e9, 1e-3).dt
-            out[f] = s.download()[:, 0]
-    d = np.abs(out[flux] - out["fp64"])
-    return {"l1_rel": float(d.sum() / np.abs(out["fp64"]).sum()), "linf_rel": float(d.max() / np.abs(out["fp64"]).max()),
+            out.append(s.download()[:, 0])
+    d = np.abs(out[1] - out[0])
+    return {"l1_rel": float(d.sum() / np.abs(out[0]).sum()), "linf_rel": float(d.max() / np.abs(out[0]).max()),
             "steps": steps, "dt": 1e-3, "mesh": n}
 
 
@@ -156,6 +148,92 @@ def copy_calibration(nbytes: int = 1 << 30) -> None:
     hip.hipFree(b)
 
 
+def measure(U0, n, flux, indicator, weights, steps, warmup):
+    """Warm-up steps, then `steps` timed steps of the workload with per-kernel event timing on the context's
+    stream: (seconds, steps, kernel times, totals before, totals after, indicator statistics or None)."""
+    ind_stats = None
+    with make_solver(n, flux, indicator, weights) as s:
+        s.upload(U0)
+        t = 0.0
+        if warmup > 0:
+            t, _, _ = s.advance(t, 1e9, max_steps=warmup)
+        tot0 = s.conserved_totals()
+        s.set_timing(True)
+        t0 = time.perf_counter()
+        t, k, _ = s.advance(t, 1e9, max_steps=steps)
+        el = time.perf_counter() - t0
+        times = s.kernel_times()
+        tot1 = s.conserved_totals()
+        if indicator == "learned":
+            last, total, off, audits = s.indicator_stats()
+            ind_stats = {"audits": audits, "switched_off": off, "tp_fp_fn_tn_all_audits": total,
+                         "missed_troubled_fraction": total[2] / max(1, total[0] + total[2]),
+                         "smooth_fraction_of_faces": (total[2] + total[3]) / max(1, sum(total))}
+    return el, k, times, tot0, tot1, ind_stats
+
+
+def other_modes(U0, n, steps, weights):
+    """The reduced-precision modes on the same workload: FP32 fluxes, and FP32 fluxes with the learned indicator
+    (when trained weights exist), each with its density error against the FP64 path."""
+    from fenice_amd.train_indicator import DEFAULT_OUT
+    out = {}
+    modes = [("fp32", "fp32", "none")]
+    if os.path.exists(weights or DEFAULT_OUT):
+        modes.append(("fp32+learned", "fp32", "learned"))
+    for name, flux, ind in modes:
+        el, k, times, _, _, st = measure(U0, n, flux, ind, weights, steps, 2)
+        r = {"value": round(n ** 3 * 3 * k / el / 1e6, 3), "ms_per_step": round(el / k * 1e3, 4), "steps": k,
+             "error_vs_fp64": error_vs_fp64(min(n, 128), flux, ind, weights)}
+        if st is not None:
+            r["indicator"] = st
+        out[name] = r
+    return out
+
+
+def side_workload(name: str) -> dict:
+    """Throughput of the other configurations of the path (parity cases, not the headline): the 1-D ensemble
+    (1024 Riemann problems of 128..8192 cells to t = 0.15, one launch after the autotuner), the isentropic vortex at
+    256^2 under MUSCL-Hancock (CFL 0.7, 200 steps, graph replay), and the Kelvin-Helmholtz layer at 2048^2
+    (WENO5-Z + SSP-RK3, 30 steps)."""
+    from fenice_amd import fvg
+    from fenice_amd.workloads import ensemble_keys, kelvin_helmholtz, kh_keys, riemann_ensemble
+    if name == "ensemble":
+        n, U, _ = riemann_ensemble()
+        with fvg.Ensemble(fvg.config_from_keys(ensemble_keys()), n) as e:
+            e.upload(U)
+            block, _ = e.tune(0.0075, 400000)
+            r = e.run(0.15, 400000)
+        stages = 2.0 * float((n.astype(np.float64) * r.steps).sum())
+        return {"workload": "1024 1-D Riemann problems, 128..8192 cells (4,243,015), MUSCL MC + HLLC + SSP-RK2, CFL 0.5, "
+                            "to t = 0.15", "block": block, "kernel_ms": round(r.ms, 2),
+                "value": round(stages / (r.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
+                "instances_done": int((r.t >= 0.15).sum()), "diverged": int(r.diverged.sum()),
+                "steps_max": int(r.steps.max()), "steps_total": int(r.steps.sum())}
+    if name == "vortex":
+        from fenice_amd.workloads import vortex_keys, vortex_state
+        with fvg.Solver(fvg.config_from_keys(vortex_keys(256, cfl="0.7"))) as s:
+            s.upload(vortex_state(256))
+            s.advance(0.0, 1e9, max_steps=5)
+            t0 = time.perf_counter()
+            _, k, _ = s.advance(0.0, 1e9, max_steps=200)
+            el = time.perf_counter() - t0
+        return {"workload": "isentropic vortex 256^2, MUSCL Van Leer + HLLC, MUSCL-Hancock, CFL 0.7, 200 steps "
+                            "(graph replay)", "value": round(65536 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
+                "ms_per_step": round(el / k * 1e3, 4), "steps": k}
+    if name == "kh":
+        n = 2048
+        with fvg.Solver(fvg.config_from_keys(kh_keys(n))) as s:
+            s.upload(kelvin_helmholtz(n))
+            s.advance(0.0, 1e9, max_steps=2)
+            t0 = time.perf_counter()
+            _, k, _ = s.advance(0.0, 1e9, max_steps=30)
+            el = time.perf_counter() - t0
+        return {"workload": "Kelvin-Helmholtz 2048^2, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic, 30 steps",
+                "value": round(n * n * 3 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
+                "ms_per_step": round(el / k * 1e3, 4), "steps": k}
+    raise ValueError(name)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,32 +246,27 @@ def main():
     ap.add_argument("--cpu-worker", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="end with a 1 GiB device copy (counter calibration)")
     ap.add_argument("--flux", default="fp64", choices=["fp64", "fp32"], help="arithmetic of reconstruction and fluxes")
+    ap.add_argument("--indicator", default="none", choices=["none", "learned"], help="troubled-face indicator (learned: fp32)")
+    ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
+    ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
+    ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh"],
+                    help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
     if a.cpu_worker:
         print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget)))
+        return
+    if a.side:
+        print(json.dumps(side_workload(a.side)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
               "see DESIGN.md for how the mesh would be split across GPUs", file=sys.stderr)
         sys.exit(2)
 
-    from fenice_amd import fvg
     n = a.n
-    cfg = fvg.config_from_keys(run_keys(n, a.flux))
     U0 = tgv_state(n)
     cells = n ** 3
-    with fvg.Solver(cfg) as s:
-        s.upload(U0)
-        t = 0.0
-        if a.warmup > 0:
-            t, _, _ = s.advance(t, 1e9, max_steps=a.warmup)
-        tot0 = s.conserved_totals()
-        s.set_timing(True)
-        t0 = time.perf_counter()
-        t, steps, s_last = s.advance(t, 1e9, max_steps=a.steps)
-        el = time.perf_counter() - t0
-        times = s.kernel_times()
-        tot1 = s.conserved_totals()
+    el, steps, times, tot0, tot1, ind_stats = measure(U0, n, a.flux, a.indicator, a.weights, a.steps, a.warmup)
     stages = 3 * steps
     value = cells * stages / el / 1e6
     # the stage as the unit: algorithmic bytes of one stage over the event-measured time of its kernels
@@ -205,7 +278,8 @@ def main():
         "metric": "Mcell-updates/s per RK stage, % HBM peak, at stated L1 error vs FP64 oracle",
         "value": round(value, 3), "unit": "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
         "ms_per_step": round(el / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.flux == "fp64" else "f32 fluxes, f64 state", "data": "synthetic (analytic Taylor-Green initial field)",
+        "dtype": "f64" if a.flux == "fp64" else ("f32 fluxes, f64 state" + (", learned indicator (bf16 MFMA)" if a.indicator == "learned" else "")),
+        "data": "synthetic (analytic Taylor-Green initial field)",
         "config": {"workload": f"3-D Taylor-Green vortex {n}^3, Navier-Stokes Re 1600 Pr 0.71 Mach 0.1, WENO5-Z + HLLC + SSP-RK3, "
                                "CFL 0.5, periodic", "cells": cells, "stages_per_step": 3},
         "l1_error_vs_fp64_oracle": 0.0 if a.flux == "fp64" else None,
@@ -216,9 +290,13 @@ def main():
         "dominant_kernel": dom[0],
         "conservation": {"mass_rel_drift": abs(tot1[0] - tot0[0]) / abs(tot0[0]), "energy_rel_drift": abs(tot1[2] - tot0[2]) / abs(tot0[2])},
     }
+    if ind_stats is not None:
+        out["indicator"] = ind_stats
     if a.flux != "fp64":
-        out["error_vs_fp64"] = error_vs_fp64(min(n, 128), a.flux)
+        out["error_vs_fp64"] = error_vs_fp64(min(n, 128), a.flux, a.indicator, a.weights)
         out["l1_error_vs_fp64_oracle"] = out["error_vs_fp64"]["l1_rel"]
+    if a.flux == "fp64" and a.indicator == "none" and not a.no_modes:
+        out["other_modes"] = other_modes(U0, n, max(10, a.steps // 3), a.weights)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_budget)
     print(json.dumps(out), flush=True)

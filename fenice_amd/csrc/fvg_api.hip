@@ -537,7 +537,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
     for (int d = 0; d < ND; ++d) {
         rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d], fe);
-        if (visc) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d], fe);
+        if (visc && !c->fp32) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d], fe);   // fp32: combined with Fc
     }
     rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
     rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);

@@ -1,5 +1,7 @@
 // Kernels of the uniform-grid residual and time step; see fvg_kernels.hpp for the pass structure.
 // Reference lines are cited per kernel; the arithmetic itself is in fvg_device.hpp.
+#include <type_traits>
+
 #include "fvg_kernels.hpp"
 
 namespace fvg {
@@ -38,6 +40,11 @@ template <int NV, typename T = double> __device__ __forceinline__ void load_prim
     const T* W = (const T*)p.W;
 #pragma unroll
     for (int q = 0; q < NV; ++q) w[q] = W[q * p.nc + c];
+}
+
+template <int NV, typename T> __device__ __forceinline__ void to_t(const double* a, T* b) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) b[q] = T(a[q]);
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs of the chip, each with its own L2. The logical block of a
@@ -216,49 +223,103 @@ __device__ __forceinline__ void visc_pad_prim(const KP& p, int i, int j, int k, 
 // k_cells: primitives; with viscosity the cell velocity gradients and the viscous speed bound
 // (Physics.cpp:44-76 2-D, 520-555 and 733-741 3-D; ViscousBound.hpp:16-24)
 // ------------------------------------------------------------------------------------------------------------
+// Viscous cell gradients G (velocity gradients from central differences of the conservative padded state) and the
+// cell's viscous speed bound (Physics.cpp:44-76 2-D, 520-555 and 733-741 3-D; ViscousBound.hpp:16-24), FP64 in the
+// reference's operation order.
+template <int NV>
+__device__ __forceinline__ double cell_gradients_f64(const KP& p, long long c, const double* w) {
+    const int i = (int)(c % p.nx);
+    const long long r = c / p.nx;
+    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+    const double rho = w[0];
+    constexpr int ND = NV - 2;
+    double grad[ND][ND + 1];   // grad[b][slot]: slots 0 (rho) and the momenta 1..ND
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        double lo[NV], hi[NV];
+        const int o0[3] = {b == 0 ? -1 : 0, b == 1 ? -1 : 0, b == 2 ? -1 : 0};
+        visc_pad_cons<NV>(p, i + o0[0], j + o0[1], k + o0[2], lo);
+        visc_pad_cons<NV>(p, i - o0[0], j - o0[1], k - o0[2], hi);
+        const double h2 = 2.0 * hstep(p, b);
+        grad[b][0] = (hi[0] - lo[0]) / h2;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) grad[b][a + 1] = (hi[mom(a)] - lo[mom(a)]) / h2;
+    }
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const double ua = w[mom(a)];
+#pragma unroll
+        for (int b = 0; b < ND; ++b) ((double*)p.G)[(a * ND + b) * p.nc + c] = (grad[b][a + 1] - ua * grad[b][0]) / rho;
+    }
+    const double g = p.rs.eos.gamma;
+    const double nu = smax((4.0 / 3.0) * (p.mu / rho), (p.k_cond * (g - 1.0)) / rho) + 1e-12;
+    double mv = fabs(w[1]);
+    if (mv < fabs(w[2])) mv = fabs(w[2]);
+    if constexpr (NV == 5)
+        if (mv < fabs(w[4])) mv = fabs(w[4]);
+    return mv + sound_speed(p.rs.eos, w[3], rho) + 2.0 * p.Ldt * p.inv_h2 * nu;
+}
+
+// The same in float for the FP32-flux mode: products with the reciprocals of 2h and rho in place of divisions.
+template <int NV>
+__device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, const float* w) {
+    const int i = (int)(c % p.nx);
+    const long long r = c / p.nx;
+    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+    const float rho = w[0], inv_rho = __builtin_amdgcn_rcpf(rho);
+    constexpr int ND = NV - 2;
+    float grad[ND][ND + 1];
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        double lo[NV], hi[NV];
+        const int o0[3] = {b == 0 ? -1 : 0, b == 1 ? -1 : 0, b == 2 ? -1 : 0};
+        visc_pad_cons<NV>(p, i + o0[0], j + o0[1], k + o0[2], lo);
+        visc_pad_cons<NV>(p, i - o0[0], j - o0[1], k - o0[2], hi);
+        const float inv2h = float(0.5 / hstep(p, b));
+        grad[b][0] = float(hi[0] - lo[0]) * inv2h;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) grad[b][a + 1] = float(hi[mom(a)] - lo[mom(a)]) * inv2h;
+    }
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float ua = w[mom(a)];
+#pragma unroll
+        for (int b = 0; b < ND; ++b) ((float*)p.G)[(a * ND + b) * p.nc + c] = (grad[b][a + 1] - ua * grad[b][0]) * inv_rho;
+    }
+    const double g = p.rs.eos.gamma;
+    const double rd = (double)rho;
+    const double nu = smax((4.0 / 3.0) * (p.mu / rd), (p.k_cond * (g - 1.0)) / rd) + 1e-12;
+    float mv = fabsf(w[1]);
+    if (mv < fabsf(w[2])) mv = fabsf(w[2]);
+    if constexpr (NV == 5)
+        if (mv < fabsf(w[4])) mv = fabsf(w[4]);
+    return double(mv) + double(sound_speed(p.rs.eos, w[3], rho)) + 2.0 * p.Ldt * p.inv_h2 * nu;
+}
+
 template <int NV, typename T = double>
 __global__ __launch_bounds__(256) void k_cells(KP p) {
     const long long c = xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
     if (c < p.nc && !p.st->done) {
-        double u[NV], w[NV];
-        load_cons<NV>(p, c, u);
-        c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
-        T* Wo = (T*)p.W;
+        if constexpr (std::is_same<T, float>::value) {
+            // FP32 fluxes: primitives and viscous cell gradients in float (the state stays FP64)
+            double u64[NV];
+            float u[NV], w[NV];
+            load_cons<NV>(p, c, u64);
+            to_t<NV>(u64, u);
+            c2p<NV>(p.rs.eos, u, w, (float*)nullptr);
+            float* Wo = (float*)p.W;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = T(w[q]);
-        if (p.viscous) {
-            const int i = (int)(c % p.nx);
-            const long long r = c / p.nx;
-            const int j = (int)(r % p.ny), k = (int)(r / p.ny);
-            const double rho = w[0];
-            constexpr int ND = NV - 2;
-            double grad[ND][ND + 1];   // grad[b][slot]: slots 0 (rho) and the momenta 1..ND
+            for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = w[q];
+            if (p.viscous) local = smax(local, cell_gradients_f32<NV>(p, c, w));
+        } else {
+            double u[NV], w[NV];
+            load_cons<NV>(p, c, u);
+            c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
+            double* Wo = (double*)p.W;
 #pragma unroll
-            for (int b = 0; b < ND; ++b) {
-                double lo[NV], hi[NV];
-                const int o0[3] = {b == 0 ? -1 : 0, b == 1 ? -1 : 0, b == 2 ? -1 : 0};
-                visc_pad_cons<NV>(p, i + o0[0], j + o0[1], k + o0[2], lo);
-                visc_pad_cons<NV>(p, i - o0[0], j - o0[1], k - o0[2], hi);
-                const double h2 = 2.0 * hstep(p, b);
-                grad[b][0] = (hi[0] - lo[0]) / h2;
-#pragma unroll
-                for (int a = 0; a < ND; ++a) grad[b][a + 1] = (hi[mom(a)] - lo[mom(a)]) / h2;
-            }
-#pragma unroll
-            for (int a = 0; a < ND; ++a) {
-                const double ua = w[mom(a)];
-#pragma unroll
-                for (int b = 0; b < ND; ++b) ((T*)p.G)[(a * ND + b) * p.nc + c] = T((grad[b][a + 1] - ua * grad[b][0]) / rho);
-            }
-            const double g = p.rs.eos.gamma;
-            const double nu = smax((4.0 / 3.0) * (p.mu / rho), (p.k_cond * (g - 1.0)) / rho) + 1e-12;
-            double mv = fabs(w[1]);
-            if (mv < fabs(w[2])) mv = fabs(w[2]);
-            if constexpr (NV == 5)
-                if (mv < fabs(w[4])) mv = fabs(w[4]);
-            const double s = mv + sound_speed(p.rs.eos, w[3], rho) + 2.0 * p.Ldt * p.inv_h2 * nu;
-            local = smax(local, s);
+            for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = w[q];
+            if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, w));
         }
     }
     if (p.viscous) block_max_to(local, &p.st->s_visc);
@@ -502,9 +563,40 @@ __device__ __forceinline__ void hancock_outside(const KP& p, int side, int i, in
 }
 
 // double-precision boundary states into the flux arithmetic type
-template <int NV, typename T> __device__ __forceinline__ void to_t(const double* a, T* b) {
+
+__device__ __forceinline__ double area_of(const KP& p, int d) { return d == 0 ? p.Ax : (d == 1 ? p.Ay : p.Az); }
+
+// The fluxes of face f normal to D. FP64: the convective flux and (with viscosity) the viscous flux in their own
+// planes, combined by k_update in the reference's order. FP32 fluxes: one combined plane set, convective flux
+// times the face area minus the viscous flux, so the update reads each face once.
+template <int NV, int D, typename T>
+__device__ __forceinline__ void store_face(const KP& p, long long f, int i, int j, int k, int t, int n, const T* F) {
+    T* Fc = (T*)p.Fc[D];
+    const long long ps = p.nfa[D];
+    if constexpr (std::is_same<T, float>::value) {
+        const float A = float(area_of(p, D));
+        float G[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) b[q] = T(a[q]);
+        for (int q = 0; q < NV; ++q) G[q] = F[q] * A;
+        if (p.viscous) {
+            float v[NV - 1];
+            viscous_face<NV, D, float>(p, i, j, k, t, n, v);
+#pragma unroll
+            for (int q = 1; q < NV; ++q) G[q] -= v[q - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = G[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
+        if (p.viscous) {
+            T v[NV - 1];
+            viscous_face<NV, D, T>(p, i, j, k, t, n, v);
+            T* Fv = (T*)p.Fv[D];
+#pragma unroll
+            for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
+        }
+    }
 }
 
 template <int NV, int D, int MODE, typename T = double>
@@ -593,17 +685,7 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
         T F[NV], sm;
         face_flux<NV>(p.rs, UL, UR, D, F, sm);
         if (double(sm) > local) local = double(sm);
-        T* Fc = (T*)p.Fc[D];
-        const long long ps = p.nfa[D];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
-        if (p.viscous) {
-            T v[NV - 1];
-            viscous_face<NV, D, T>(p, i, j, k, t, n, v);
-            T* Fv = (T*)p.Fv[D];
-#pragma unroll
-            for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
-        }
+        store_face<NV, D, T>(p, f, i, j, k, t, n, F);
     }
     block_max_to(local, &p.st->s_conv);
 }
@@ -688,17 +770,7 @@ __global__ __launch_bounds__(256) void k_faces_learned(KP p) {
         float F[NV], sm;
         face_flux<NV>(p.rs, UL, UR, D, F, sm);
         if (double(sm) > local) local = double(sm);
-        float* Fc = (float*)p.Fc[D];
-        const long long ps = p.nfa[D];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) Fc[q * ps + f] = F[q];
-        if (p.viscous) {
-            float v[NV - 1];
-            viscous_face<NV, D, float>(p, i, j, k, t, n, v);
-            float* Fv = (float*)p.Fv[D];
-#pragma unroll
-            for (int q = 0; q < NV - 1; ++q) Fv[q * ps + f] = v[q];
-        }
+        store_face<NV, D, float>(p, f, i, j, k, t, n, F);
     }
     block_max_to(local, &p.st->s_conv);
 }
@@ -781,6 +853,21 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
     const long long fz = c;
     const long long fyt = fy + p.nx, fzk = fz + (long long)p.nx * p.ny;
     double R[NV];
+    if constexpr (std::is_same<T, float>::value) {
+        // combined face fluxes (store_face); the differences of two floats are exact in double, so the balance
+        // still telescopes to FP64 round-off
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const float* Fx = (const float*)p.Fc[0] + q * p.nfa[0];
+            const float* Fy = (const float*)p.Fc[1] + q * p.nfa[1];
+            double d = (double(Fx[fx + 1]) - double(Fx[fx])) + (double(Fy[fyt]) - double(Fy[fy]));
+            if constexpr (NV == 5) {
+                const float* Fz = (const float*)p.Fc[2] + q * p.nfa[2];
+                d += double(Fz[fzk]) - double(Fz[fz]);
+            }
+            R[q] = -d * p.inv_vol;
+        }
+    } else {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         const T* Fx = (const T*)p.Fc[0] + q * p.nfa[0];
@@ -810,6 +897,7 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
             }
             R[q] = R[q] + v;
         }
+    }
     }
     const double dt = p.st->dt;
 #pragma unroll

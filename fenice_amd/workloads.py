@@ -5,6 +5,7 @@ every run (device, oracle, reference) starts from the same bits.
   N_i ~ U{nmin..nmax}, rho_L, rho_R ~ logU[0.1, 10], p_L, p_R ~ logU[0.01, 1000], u_L, u_R ~ U[-2, 2],
   x0 ~ U[0.3, 0.7], drawn in that order per instance from one SplitMix64 stream (seed 1024 for the 1024-instance
   ensemble).
+- `taylor_green`: the 3-D Taylor-Green vortex at Mach 0.1 (the benchmark's workload).
 - `kelvin_helmholtz`: the periodic shear layer on [0, 1]^2 (rho 2, u -0.5 for |y - 0.5| < 0.25, else rho 1,
   u 0.5; p 2.5; v = 0.01 sin(4 pi x) + U(-1e-3, 1e-3)), the noise from SplitMix64(seed) indexed by cell.
 """
@@ -101,3 +102,58 @@ def kh_keys(n: int) -> dict:
     return dict(nx=str(n), ny=str(n), nz="1", Lx="1.0", Ly="1.0", gamma="1.4", use_viscous="false", bc_xmin="periodic",
                 bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", riemann_solver="HLLC", reconstruction="weno5",
                 time_scheme="SSP Runge-Kutta 3", cfl="0.5")
+
+
+def taylor_green(n: int, gamma: float = 1.4, mach: float = 0.1) -> np.ndarray:
+    """(n^3, 5) conservative state of the 3-D Taylor-Green vortex on [0, 2 pi]^3 (cpp/src/TestCases.cpp:262-271):
+    rho 1, u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+    p = 1 / (gamma Ma^2) + (cos 2x + cos 2y)(cos 2z + 2) / 16."""
+    h = 2.0 * np.pi / n
+    x = (np.arange(n) + 0.5) * h
+    z3, y3, x3 = np.meshgrid(x, x, x, indexing="ij")
+    u = np.sin(x3) * np.cos(y3) * np.cos(z3)
+    v = -np.cos(x3) * np.sin(y3) * np.cos(z3)
+    p = 1.0 / (gamma * mach * mach) + (np.cos(2 * x3) + np.cos(2 * y3)) * (np.cos(2 * z3) + 2.0) / 16.0
+    U = np.empty((n ** 3, 5))
+    U[:, 0] = 1.0
+    U[:, 1] = u.ravel()
+    U[:, 2] = v.ravel()
+    U[:, 3] = p.ravel() / (gamma - 1.0) + 0.5 * (u * u + v * v).ravel()
+    U[:, 4] = 0.0
+    return U
+
+
+def taylor_green_keys(n: int, flux: str = "fp64") -> dict:
+    """Navier-Stokes at Re 1600 (mu 6.25e-4), Pr 0.71, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic."""
+    L = "6.283185307179586"
+    return dict(nx=n, ny=n, nz=n, Lx=L, Ly=L, Lz=L, gamma=1.4, use_viscous="true", mu="6.25e-4", pr="0.71",
+                bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", bc_zmin="periodic",
+                bc_zmax="periodic", riemann_solver="HLLC", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5",
+                flux_precision=flux)
+
+
+def vortex_state(n: int, L: float = 10.0, gamma: float = 1.4) -> np.ndarray:
+    """(n*n, 4) conservative state of the isentropic vortex (beta 5, free stream (1, 1), centre (L/2, L/2)) on an n x n
+    periodic square (cpp/src/TestCases.cpp:284-295)."""
+    h = L / n
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    dx = (i.ravel() + 0.5) * h - 0.5 * L
+    dy = (j.ravel() + 0.5) * h - 0.5 * L
+    dx = dx - L * np.round(dx / L)
+    dy = dy - L * np.round(dy / L)
+    r2 = dx * dx + dy * dy
+    g = 5.0 / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (gamma - 1.0) * 25.0 / (8.0 * gamma * np.pi * np.pi) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    u, v, p = 1.0 - g * dy, 1.0 + g * dx, rho * T
+    U = np.zeros((n * n, 4))
+    U[:, 0], U[:, 1], U[:, 2] = rho, rho * u, rho * v
+    U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def vortex_keys(n: int, cfl: str = "0.8", time_scheme: str = "MUSCL-Hancock") -> dict:
+    """MUSCL (Van Leer) + HLLC, periodic [0, 10]^2."""
+    return dict(nx=str(n), ny=str(n), nz="1", Lx="10.0", Ly="10.0", gamma="1.4", use_viscous="false", bc_xmin="periodic",
+                bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", riemann_solver="HLLC", reconstruction="muscl",
+                muscl_limiter="Van Leer", time_scheme=time_scheme, cfl=cfl, boundary_order="1")

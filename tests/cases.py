@@ -116,34 +116,15 @@ def state_for(kind, keys, seed=7):
 VORTEX_HANCOCK_256 = dict(nx="256", ny="256", nz="1", Lx="10.0", Ly="10.0", gamma="1.4", use_viscous="false",
                           bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic",
                           riemann_solver="HLLC", reconstruction="muscl", muscl_limiter="Van Leer",
-                          time_scheme="MUSCL-Hancock", cfl="0.8", boundary_order="1")
+                          time_scheme="MUSCL-Hancock", cfl="0.8", boundary_order="1")   # = workloads.vortex_keys(256)
 VORTEX_DIVERGES_AT = 65
 
 
 def vortex_state(n, L=10.0, gamma=1.4):
-    """Isentropic vortex (beta 5, free stream (1, 1), centre (L/2, L/2)) on an n x n periodic square, as the reference
-    defines it (cpp/src/TestCases.cpp:284-295), converted to conserved variables; evaluated here with numpy, so both
-    sides of a parity test start from the same host bits."""
-    h = L / n
-    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
-    x = (i.ravel() + 0.5) * h
-    y = (j.ravel() + 0.5) * h
-    dx = x - 0.5 * L
-    dx = dx - L * np.round(dx / L)
-    dy = y - 0.5 * L
-    dy = dy - L * np.round(dy / L)
-    r2 = dx * dx + dy * dy
-    g = 5.0 / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
-    T = 1.0 - (gamma - 1.0) * 25.0 / (8.0 * gamma * np.pi * np.pi) * np.exp(1.0 - r2)
-    rho = T ** (1.0 / (gamma - 1.0))
-    u = 1.0 - g * dy
-    v = 1.0 + g * dx
-    p = rho * T
+    """The isentropic vortex (fenice_amd.workloads.vortex_state) with the reference's 5-slot State layout."""
+    from fenice_amd.workloads import vortex_state as v4
     U = np.zeros((n * n, 5))
-    U[:, 0] = rho
-    U[:, 1] = rho * u
-    U[:, 2] = rho * v
-    U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    U[:, :4] = v4(n, L, gamma)
     return U
 
 
