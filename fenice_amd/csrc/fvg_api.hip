@@ -22,6 +22,7 @@ template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV> __global__ void k_totals_partial(const double*, long long, double*);
 template <int NV, int D> __global__ void k_faces_learned(KP);
+template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
 template <int NV, int D> __global__ void k_audit(KP);
 template <int NV, int D> __global__ void k_samples(KP, float*, float*);
 __global__ void k_indicator_control(KP, int);
@@ -36,11 +37,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kGraphSteps = 4;   // steps per captured graph of fvg_advance
+constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along x, 16 along z
 constexpr int kTotalsBlocks = 1024;
 thread_local std::string g_create_error;
 
-enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_N };
-const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control"};
+enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_N };
+const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_zupdate"};
 
 struct Err {
     int code;
@@ -78,6 +80,7 @@ struct fvg_ctx {
     MlpDev* mlp = nullptr;
     IndState* ind = nullptr;
     bool graphs = true;                // fvg_advance replays a captured graph of steps
+    bool fused = true;                 // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
     hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
 };
 
@@ -247,6 +250,70 @@ int enqueue_update(fvg_ctx* c, int upd, const double* U0, const double* Ut, doub
     return c->NV == 5 ? enqueue_update_nv<5, double>(c, upd, U0, Ut, out) : enqueue_update_nv<4, double>(c, upd, U0, Ut, out);
 }
 
+// A Runge-Kutta stage after the first (dt known): cells, the faces of every axis but the last, then the last axis's
+// faces (z in 3-D, y in 2-D) fused with the update.
+template <int NV, typename T>
+int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
+    const KP p = make_kp(c, Uin);
+    int rc;
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+        return rc;
+    for (int d = 0; d < (NV == 5 ? 2 : 1); ++d) {
+        rc = launch(c, FAM_FACES_X + d, [&] {
+            const dim3 g(grid_for(c->nfa[d])), b(kBlock);
+            if constexpr (std::is_same<T, float>::value) {
+                if (mode == MODE_LEARNED) {
+                    if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0>), g, b, 0, c->stream, p);
+                    else hipLaunchKernelGGL((k_faces_learned<NV, 1>), g, b, 0, c->stream, p);
+                    return;
+                }
+            }
+            if (mode == MODE_PLAIN) {
+                if (d == 0) hipLaunchKernelGGL((k_faces<NV, 0, MODE_PLAIN, T>), g, b, 0, c->stream, p);
+                else hipLaunchKernelGGL((k_faces<NV, 1, MODE_PLAIN, T>), g, b, 0, c->stream, p);
+            } else {
+                if (d == 0) hipLaunchKernelGGL((k_faces<NV, 0, MODE_FIRST_ORDER, T>), g, b, 0, c->stream, p);
+                else hipLaunchKernelGGL((k_faces<NV, 1, MODE_FIRST_ORDER, T>), g, b, 0, c->stream, p);
+            }
+        });
+        if (rc) return rc;
+    }
+    const long long nit = (c->cfg.nx + kZTileI - 1) / kZTileI;
+    const long long nlast = NV == 5 ? c->cfg.nz : c->cfg.ny;
+    const long long blocks = nit * (NV == 5 ? c->cfg.ny : 1) * ((nlast + kZTileK - 2) / (kZTileK - 1));
+    const dim3 g((unsigned)blocks), b(kZTileI * kZTileK);
+    return launch(c, FAM_ZUPDATE, [&] {
+#define ZU(M, U) hipLaunchKernelGGL((k_zupdate<NV, M, U, T>), g, b, 0, c->stream, p, U0, Ut, out)
+#define ZU_MODE(M)                           \
+    switch (upd) {                           \
+    case UPD_RK2_2: ZU(M, UPD_RK2_2); break; \
+    case UPD_RK3_2: ZU(M, UPD_RK3_2); break; \
+    default: ZU(M, UPD_RK3_3); break;        \
+    }
+        if constexpr (std::is_same<T, float>::value) {
+            if (mode == MODE_LEARNED) {
+                ZU_MODE(MODE_LEARNED)
+                return;
+            }
+        }
+        if (mode == MODE_PLAIN) {
+            ZU_MODE(MODE_PLAIN)
+        } else {
+            ZU_MODE(MODE_FIRST_ORDER)
+        }
+#undef ZU_MODE
+#undef ZU
+    });
+}
+
+int fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
+    if (c->NV == 5)
+        return c->fp32 ? enqueue_fused_stage<5, float>(c, Uin, mode, upd, U0, Ut, out)
+                       : enqueue_fused_stage<5, double>(c, Uin, mode, upd, U0, Ut, out);
+    return c->fp32 ? enqueue_fused_stage<4, float>(c, Uin, mode, upd, U0, Ut, out)
+                   : enqueue_fused_stage<4, double>(c, Uin, mode, upd, U0, Ut, out);
+}
+
 int plain_mode(const fvg_ctx* c) {
     if (c->learned) return MODE_LEARNED;
     return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN;
@@ -280,6 +347,19 @@ int enqueue_step(fvg_ctx* c) {
         return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->Ut))) return rc;
+    if (c->fused) {   // stages 2-3: the last axis' faces fused with the update; the SSP-RK3 stage-2 output goes to Rb
+        if (ts == FVG_TS_SSPRK2) {
+            if ((rc = fused_stage(c, c->Ut, mode, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
+        } else {
+            if ((rc = fused_stage(c, c->Ut, mode, UPD_RK3_2, c->U, c->Ut, c->Rb))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
+            if ((rc = fused_stage(c, c->Rb, mode, UPD_RK3_3, c->U, c->Rb, c->U))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
+        }
+        if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
+        return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
+    }
     if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
     if (ts == FVG_TS_SSPRK2) {
@@ -820,6 +900,16 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
     hipFree(dX);
     hipFree(dd);
     return rc;
+}
+
+int fvg_set_fused(fvg_ctx* c, int on) {
+    if (!c) return FVG_EINVAL;
+    c->fused = on != 0;
+    if (c->graph) {   // the captured graph holds the old kernel sequence
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return FVG_OK;
 }
 
 int fvg_set_graphs(fvg_ctx* c, int on) {

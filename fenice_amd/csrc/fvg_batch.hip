@@ -199,7 +199,8 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
     while (t < b.t_end && steps < b.max_steps) {
         for (int st = 1; st <= nst; ++st) {
             const double* Us = st == 1 ? b.U : b.Ut;
-            // primitives of the stage input (k_cells)
+            // primitives of the stage input (k_cells) and the wave speed of each cell's y faces
+            double local = 0.0;
             for (int i = threadIdx.x; i < n; i += BS) {
                 double u[4], w[4];
                 bload_cons(b, in, Us, i, u);
@@ -208,10 +209,11 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
                 b.W[c] = w[0];
                 b.W[b.nc + c] = w[1];
                 b.W[2 * b.nc + c] = w[3];
+                const double sy = by_speed<RECON>(b, in, Us, i);
+                if (sy > local) local = sy;
             }
             __syncthreads();
-            // x faces and the y-face wave speed of each cell (k_faces)
-            double local = 0.0;
+            // x faces (k_faces)
             for (int f = threadIdx.x; f <= n; f += BS) {
                 double F[4], sm;
                 bface<RECON>(b, in, Us, f, F, sm);
@@ -220,10 +222,6 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
                 b.F[x] = F[0];
                 b.F[b.nf + x] = F[1];
                 b.F[2 * b.nf + x] = F[3];
-                if (f < n) {
-                    const double sy = by_speed<RECON>(b, in, Us, f);
-                    if (sy > local) local = sy;
-                }
             }
             const double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
             // the scalar part of the scheme (k_control)

@@ -302,12 +302,14 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
     double local = 0.0;
     if (c < p.nc && !p.st->done) {
         if constexpr (std::is_same<T, float>::value) {
-            // FP32 fluxes: primitives and viscous cell gradients in float (the state stays FP64)
-            double u64[NV];
-            float u[NV], w[NV];
-            load_cons<NV>(p, c, u64);
-            to_t<NV>(u64, u);
-            c2p<NV>(p.rs.eos, u, w, (float*)nullptr);
+            // FP32 fluxes: the primitives are the FP64 ones rounded to float (the viscous ghosts of a periodic side
+            // round the same FP64 primitives, so the two faces of a periodic seam stay identical); the viscous cell
+            // gradients are computed in float
+            double u[NV], wd[NV];
+            float w[NV];
+            load_cons<NV>(p, c, u);
+            c2p<NV>(p.rs.eos, u, wd, (double*)nullptr);
+            to_t<NV>(wd, w);
             float* Wo = (float*)p.W;
 #pragma unroll
             for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = w[q];
@@ -599,6 +601,83 @@ __device__ __forceinline__ void store_face(const KP& p, long long f, int i, int 
     }
 }
 
+// The two states of face (i, j, k) normal to D (t its coordinate along D, n the cells along D) that the Riemann
+// solver takes: reconstructed from the padded primitives, first-order at a side without a halo, or the
+// MUSCL-Hancock predicted states (Schemes.cpp:389-749, 2934-2988).
+template <int NV, int D, int MODE, typename T>
+__device__ __forceinline__ void face_states(const KP& p, int i, int j, int k, int t, int n, T* UL, T* UR) {
+    const bool lo = t == 0, hi = t == n;
+    // the cells left and right of the face (their coordinate along D is t - 1 and t)
+    const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
+    if constexpr (MODE == MODE_FIRST_ORDER) {
+        double ul[NV], ur[NV];
+        if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+        else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+        if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+        else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+        to_t<NV>(ul, UL);
+        to_t<NV>(ur, UR);
+    } else if constexpr (MODE == MODE_PLAIN) {
+        const bool upgrade = (t > 0 && t < n) || (lo && halo_of(p, 2 * D)) || (hi && halo_of(p, 2 * D + 1));
+        if (upgrade) {
+            T wl[NV], wr[NV];
+            const T we = T(p.weno_eps);
+            if (p.recon == 3) {   // WENO5-Z on six cells t-3 .. t+2
+                T c[6][NV];
+#pragma unroll
+                for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                    wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+                }
+            } else {   // four cells t-2 .. t+1
+                T c[4][NV];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 2 + s, c[s]);
+                if (p.recon == 2) {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
+                        wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        wl[q] = muscl_wl(p.lim, c[0][q], c[1][q], c[2][q]);
+                        wr[q] = muscl_wr(p.lim, c[1][q], c[2][q], c[3][q]);
+                    }
+                }
+            }
+            p2c<NV>(p.rs.eos, wl, UL);
+            p2c<NV>(p.rs.eos, wr, UR);
+        } else {
+            double ul[NV], ur[NV];
+            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+            else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+            roundtrip<NV>(p.rs.eos, ul);
+            roundtrip<NV>(p.rs.eos, ur);
+            to_t<NV>(ul, UL);
+            to_t<NV>(ur, UR);
+        }
+    } else {   // MUSCL-Hancock: the predicted states of the cells on both sides (Schemes.cpp:2934-2988)
+        if (!lo) {
+            const long long cc = cidx(p, il, jl, kl);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) UL[q] = p.HS[((long long)(2 * D + 1) * NV + q) * p.nc + cc];
+        }
+        if (!hi) {
+            const long long cc = cidx(p, i, j, k);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) UR[q] = p.HS[((long long)(2 * D) * NV + q) * p.nc + cc];
+        }
+        if (lo) hancock_outside<NV, D>(p, 2 * D, i, j, k, t, UR, UL);
+        if (hi) hancock_outside<NV, D>(p, 2 * D + 1, il, jl, kl, t, UL, UR);
+    }
+}
+
 template <int NV, int D, int MODE, typename T = double>
 __global__ __launch_bounds__(256) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
@@ -611,77 +690,8 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
         const int j = (int)(r % nj), k = (int)(r / nj);
         const int t = coord(D, i, j, k);
         const int n = ncell(p, D);
-        const bool lo = t == 0, hi = t == n;
-        // the cells left and right of the face (their coordinate along D is t - 1 and t)
-        const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
         T UL[NV], UR[NV];
-        if constexpr (MODE == MODE_FIRST_ORDER) {
-            double ul[NV], ur[NV];
-            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
-            else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
-            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
-            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
-            to_t<NV>(ul, UL);
-            to_t<NV>(ur, UR);
-        } else if constexpr (MODE == MODE_PLAIN) {
-            const bool upgrade = (t > 0 && t < n) || (lo && halo_of(p, 2 * D)) || (hi && halo_of(p, 2 * D + 1));
-            if (upgrade) {
-                T wl[NV], wr[NV];
-                const T we = T(p.weno_eps);
-                if (p.recon == 3) {   // WENO5-Z on six cells t-3 .. t+2
-                    T c[6][NV];
-#pragma unroll
-                    for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
-#pragma unroll
-                    for (int q = 0; q < NV; ++q) {
-                        wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
-                        wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
-                    }
-                } else {   // four cells t-2 .. t+1
-                    T c[4][NV];
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 2 + s, c[s]);
-                    if (p.recon == 2) {
-#pragma unroll
-                        for (int q = 0; q < NV; ++q) {
-                            wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
-                            wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
-                        }
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < NV; ++q) {
-                            wl[q] = muscl_wl(p.lim, c[0][q], c[1][q], c[2][q]);
-                            wr[q] = muscl_wr(p.lim, c[1][q], c[2][q], c[3][q]);
-                        }
-                    }
-                }
-                p2c<NV>(p.rs.eos, wl, UL);
-                p2c<NV>(p.rs.eos, wr, UR);
-            } else {
-                double ul[NV], ur[NV];
-                if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
-                else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
-                if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
-                else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
-                roundtrip<NV>(p.rs.eos, ul);
-                roundtrip<NV>(p.rs.eos, ur);
-                to_t<NV>(ul, UL);
-                to_t<NV>(ur, UR);
-            }
-        } else {   // MUSCL-Hancock: the predicted states of the cells on both sides (Schemes.cpp:2934-2988)
-            if (!lo) {
-                const long long cc = cidx(p, il, jl, kl);
-#pragma unroll
-                for (int q = 0; q < NV; ++q) UL[q] = p.HS[((long long)(2 * D + 1) * NV + q) * p.nc + cc];
-            }
-            if (!hi) {
-                const long long cc = cidx(p, i, j, k);
-#pragma unroll
-                for (int q = 0; q < NV; ++q) UR[q] = p.HS[((long long)(2 * D) * NV + q) * p.nc + cc];
-            }
-            if (lo) hancock_outside<NV, D>(p, 2 * D, i, j, k, t, UR, UL);
-            if (hi) hancock_outside<NV, D>(p, 2 * D + 1, il, jl, kl, t, UL, UR);
-        }
+        face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
         T F[NV], sm;
         face_flux<NV>(p.rs, UL, UR, D, F, sm);
         if (double(sm) > local) local = double(sm);
@@ -711,15 +721,10 @@ __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
     return (t > 0 && t < n) || (t == 0 && halo_of(p, 2 * D)) || (t == n && halo_of(p, 2 * D + 1));
 }
 
+// States and flux of face (i, j, k) normal to D through the learned indicator. Every lane of the wavefront calls
+// it (the network runs on the matrix cores); lanes with active = false only take part in the network.
 template <int NV, int D>
-__global__ __launch_bounds__(256) void k_faces_learned(KP p) {
-    if (p.st->done) return;   // uniform: the whole grid leaves
-    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
-    const long long nf = (long long)ni * nj * nk;
-    const long long f = xcd_block() * blockDim.x + threadIdx.x;
-    const bool active = f < nf;
-    int i = 0, j = 0, k = 0, t = 0;
-    if (active) face_coords<NV, D>(p, f, i, j, k, t);
+__device__ __forceinline__ void learned_face(const KP& p, bool active, int i, int j, int k, int t, int n, float* F, float& sm) {
     const bool upgrade = active && face_upgrade<NV, D>(p, t);
     float c[6][NV];
     float x[16];
@@ -730,45 +735,58 @@ __global__ __launch_bounds__(256) void k_faces_learned(KP p) {
         for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
         face_features<NV>(c, float(p.weno_eps), x);
     }
-    const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
+    const float z = mlp_logit_wave(p.mlp, x);
+    if (!active) return;
+    const bool lo = t == 0, hi = t == n;
+    const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
+    float UL[NV], UR[NV];
+    if (upgrade) {
+        const bool smooth = z <= p.mlp->thr && !p.ind->off;
+        float wl[NV], wr[NV];
+        const float we = float(p.weno_eps);
+        if (smooth) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                wl[q] = upwind5(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q]);
+                wr[q] = upwind5(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+            }
+        }
+        p2c<NV>(p.rs.eos, wl, UL);
+        p2c<NV>(p.rs.eos, wr, UR);
+    } else {
+        double ul[NV], ur[NV];
+        if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
+        else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
+        if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
+        else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
+        roundtrip<NV>(p.rs.eos, ul);
+        roundtrip<NV>(p.rs.eos, ur);
+        to_t<NV>(ul, UL);
+        to_t<NV>(ur, UR);
+    }
+    face_flux<NV>(p.rs, UL, UR, D, F, sm);
+}
+
+template <int NV, int D>
+__global__ __launch_bounds__(256) void k_faces_learned(KP p) {
+    if (p.st->done) return;   // uniform: the whole grid leaves
+    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
+    const long long nf = (long long)ni * nj * nk;
+    const long long f = xcd_block() * blockDim.x + threadIdx.x;
+    const bool active = f < nf;
+    int i = 0, j = 0, k = 0, t = 0;
+    if (active) face_coords<NV, D>(p, f, i, j, k, t);
+    const int n = ncell(p, D);
+    float F[NV], sm = 0.0f;
+    learned_face<NV, D>(p, active, i, j, k, t, n, F, sm);
     double local = 0.0;
     if (active) {
-        const int n = ncell(p, D);
-        const bool lo = t == 0, hi = t == n;
-        const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
-        float UL[NV], UR[NV];
-        if (upgrade) {
-            const bool smooth = z <= p.mlp->thr && !p.ind->off;
-            float wl[NV], wr[NV];
-            const float we = float(p.weno_eps);
-            if (smooth) {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) {
-                    wl[q] = upwind5(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q]);
-                    wr[q] = upwind5(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q]);
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) {
-                    wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
-                    wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
-                }
-            }
-            p2c<NV>(p.rs.eos, wl, UL);
-            p2c<NV>(p.rs.eos, wr, UR);
-        } else {
-            double ul[NV], ur[NV];
-            if (!lo) load_cons<NV>(p, cidx(p, il, jl, kl), ul);
-            else outside_first_order<NV>(p, D, 2 * D, i, j, k, ul);
-            if (!hi) load_cons<NV>(p, cidx(p, i, j, k), ur);
-            else outside_first_order<NV>(p, D, 2 * D + 1, il, jl, kl, ur);
-            roundtrip<NV>(p.rs.eos, ul);
-            roundtrip<NV>(p.rs.eos, ur);
-            to_t<NV>(ul, UL);
-            to_t<NV>(ur, UR);
-        }
-        float F[NV], sm;
-        face_flux<NV>(p.rs, UL, UR, D, F, sm);
         if (double(sm) > local) local = double(sm);
         store_face<NV, D, float>(p, f, i, j, k, t, n, F);
     }
@@ -841,6 +859,24 @@ __global__ __launch_bounds__(256) void k_samples(KP p, float* __restrict__ X, fl
 // (Physics.cpp:196-204, 746-752) and the stage combination of the time scheme (TimeScheme.cpp:27-59, 150-156,
 // 213-219), one cell per thread.
 // ------------------------------------------------------------------------------------------------------------
+// The stage combination of the time scheme (TimeScheme.cpp:27-59, 150-156, 213-219) for cell c.
+template <int NV, int UPD>
+__device__ __forceinline__ void combine(const KP& p, long long c, const double* R, const double* __restrict__ U0,
+                                        const double* __restrict__ Ut, double* __restrict__ out) {
+    const double dt = p.st->dt;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const long long x = q * p.nc + c;
+        double o;
+        if constexpr (UPD == UPD_RESIDUAL) o = R[q];
+        else if constexpr (UPD == UPD_AXPY) o = U0[x] + dt * R[q];
+        else if constexpr (UPD == UPD_RK2_2) o = 0.5 * U0[x] + 0.5 * (Ut[x] + dt * R[q]);
+        else if constexpr (UPD == UPD_RK3_2) o = 0.75 * U0[x] + 0.25 * (Ut[x] + dt * R[q]);
+        else o = (1.0 / 3.0) * U0[x] + (2.0 / 3.0) * (Ut[x] + dt * R[q]);
+        out[x] = o;
+    }
+}
+
 template <int NV, int UPD, typename T = double>
 __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
     const long long c = xcd_block() * blockDim.x + threadIdx.x;
@@ -899,18 +935,132 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
         }
     }
     }
-    const double dt = p.st->dt;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        const long long x = q * p.nc + c;
-        double o;
-        if constexpr (UPD == UPD_RESIDUAL) o = R[q];
-        else if constexpr (UPD == UPD_AXPY) o = U0[x] + dt * R[q];
-        else if constexpr (UPD == UPD_RK2_2) o = 0.5 * U0[x] + 0.5 * (Ut[x] + dt * R[q]);
-        else if constexpr (UPD == UPD_RK3_2) o = 0.75 * U0[x] + 0.25 * (Ut[x] + dt * R[q]);
-        else o = (1.0 / 3.0) * U0[x] + (2.0 / 3.0) * (Ut[x] + dt * R[q]);
-        out[x] = o;
+    combine<NV, UPD>(p, c, R, U0, Ut, out);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// k_zupdate: the faces of the last axis (z in 3-D, y in 2-D) and the cell update in one pass, for the stages
+// whose dt is already known (stage 2 of SSP-RK2, stages 2 and 3 of SSP-RK3). A block of 1024 threads takes 64
+// x-positions by 16 faces along the last axis: every thread computes one face into LDS, then 15 of the 16 levels
+// update their cell from the face fluxes of the earlier passes and the two fluxes in LDS, in k_update's order. The
+// tile's top face is also the bottom face of the tile above (the same bits, counted twice in the maximum, which
+// does not change it). The output never aliases the stage input, so no thread reads a cell another thread has
+// already updated.
+// ------------------------------------------------------------------------------------------------------------
+constexpr int ZT_I = 64, ZT_K = 16;   // k_zupdate tile: 64 faces along x by 16 along the last axis (15 cells updated)
+
+// face (i, j, k) of the last axis D (z in 3-D, y in 2-D) with its coordinate kf along D
+template <int NV, int MODE, typename T>
+__device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j, int k, int kf, T* Fc, T* Fv, double& local) {
+    constexpr int D = NV == 5 ? 2 : 1;
+    const int n = ncell(p, D);
+    T sm = T(0);
+    if constexpr (MODE == MODE_LEARNED) {
+        learned_face<NV, D>(p, active, i, j, k, kf, n, Fc, sm);
+    } else {
+        if (!active) return;
+        T UL[NV], UR[NV];
+        face_states<NV, D, MODE, T>(p, i, j, k, kf, n, UL, UR);
+        face_flux<NV>(p.rs, UL, UR, D, Fc, sm);
     }
+    if (!active) return;
+    if (double(sm) > local) local = double(sm);
+    if (p.viscous) viscous_face<NV, D, T>(p, i, j, k, kf, n, Fv);
+    if constexpr (std::is_same<T, float>::value) {   // the combined flux of store_face
+        const float A = float(area_of(p, D));
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Fc[q] = Fc[q] * A;
+        if (p.viscous)
+#pragma unroll
+            for (int q = 1; q < NV; ++q) Fc[q] -= Fv[q - 1];
+    }
+}
+
+template <int NV, int MODE, int UPD, typename T>
+__global__ __launch_bounds__(1024) void k_zupdate(KP p, const double* __restrict__ U0, const double* __restrict__ Ut,
+                                                  double* __restrict__ out) {
+    constexpr int D = NV == 5 ? 2 : 1;
+    __shared__ T sF[ZT_K][NV][ZT_I];
+    __shared__ T sV[ZT_K][NV - 1][ZT_I];
+    if (p.st->done) return;   // uniform
+    const int li = threadIdx.x % ZT_I, lk = threadIdx.x / ZT_I;
+    const int nit = (p.nx + ZT_I - 1) / ZT_I;
+    const long long b = xcd_block();
+    const int it = (int)(b % nit);
+    const long long r = b / nit;
+    // 3-D: tiles (i, j, z); 2-D: tiles (i, y) of the single plane
+    const int j = D == 2 ? (int)(r % p.ny) : 0, kt = D == 2 ? (int)(r / p.ny) : (int)r;
+    const int n = ncell(p, D);
+    const int i = it * ZT_I + li, kf = kt * (ZT_K - 1) + lk;   // this thread's face along D
+    const bool af = i < p.nx && kf <= n;
+    const int fj = D == 2 ? j : kf, fk = D == 2 ? kf : 0;     // the face's (j, k)
+    double local = 0.0;
+    T Ft[NV], Vt[NV - 1];
+#pragma unroll
+    for (int q = 0; q < NV - 1; ++q) Vt[q] = T(0);
+    last_face<NV, MODE, T>(p, af, i, fj, fk, kf, Ft, Vt, local);   // a wavefront holds one level and 64 x-positions
+    if (af) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sF[lk][q][li] = Ft[q];
+#pragma unroll
+        for (int q = 0; q < NV - 1; ++q) sV[lk][q][li] = Vt[q];
+    }
+    __syncthreads();
+    if (lk < ZT_K - 1 && af && kf < n) {   // the cell between faces lk and lk + 1 of the tile
+        const int cj = fj, ck = fk;
+        const long long c = cidx(p, i, cj, ck);
+        const long long fx = ((long long)ck * p.ny + cj) * (p.nx + 1) + i;
+        const long long fy = ((long long)ck * (p.ny + 1) + cj) * p.nx + i;
+        const long long fyt = fy + p.nx;
+        double R[NV];
+        if constexpr (std::is_same<T, float>::value) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const float* Fx = (const float*)p.Fc[0] + q * p.nfa[0];
+                double d = double(Fx[fx + 1]) - double(Fx[fx]);
+                if constexpr (D == 2) {
+                    const float* Fy = (const float*)p.Fc[1] + q * p.nfa[1];
+                    d = d + (double(Fy[fyt]) - double(Fy[fy]));
+                }
+                d += double(sF[lk + 1][q][li]) - double(sF[lk][q][li]);
+                R[q] = -d * p.inv_vol;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const double* Fx = (const double*)p.Fc[0] + q * p.nfa[0];
+                const double fl = Fx[fx], fr = Fx[fx + 1];
+                const double fa = sF[lk][q][li], fz = sF[lk + 1][q][li];
+                if constexpr (D == 2) {
+                    const double* Fy = (const double*)p.Fc[1] + q * p.nfa[1];
+                    const double fb = Fy[fy], ft = Fy[fyt];
+                    R[q] = (((0.0 - (fr * p.Ax - fl * p.Ax)) - (ft * p.Ay - fb * p.Ay)) - (fz * p.Az - fa * p.Az)) * p.inv_vol;
+                } else {
+                    R[q] = ((0.0 - (fr * p.Ax - fl * p.Ax)) - (fz * p.Ay - fa * p.Ay)) * p.inv_vol;
+                }
+            }
+            if (p.viscous) {
+                R[0] = R[0] + 0.0;
+#pragma unroll
+                for (int q = 1; q < NV; ++q) {
+                    const double* Fx = (const double*)p.Fv[0] + (q - 1) * p.nfa[0];
+                    const double fl = Fx[fx], fr = Fx[fx + 1];
+                    const double va = sV[lk][q - 1][li], vz = sV[lk + 1][q - 1][li];
+                    double v;
+                    if constexpr (D == 2) {
+                        const double* Fy = (const double*)p.Fv[1] + (q - 1) * p.nfa[1];
+                        const double fb = Fy[fy], ft = Fy[fyt];
+                        v = (((0.0 + (fr - fl)) + (ft - fb)) + (vz - va)) * p.inv_vol;
+                    } else {
+                        v = ((0.0 + (fr - fl)) + (vz - va)) * p.inv_vol;
+                    }
+                    R[q] = R[q] + v;
+                }
+            }
+        }
+        combine<NV, UPD>(p, c, R, U0, Ut, out);
+    }
+    block_max_to(local, &p.st->s_conv);
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1113,6 +1263,21 @@ template __global__ void k_samples<4, 1>(KP, float*, float*);
 template __global__ void k_samples<5, 0>(KP, float*, float*);
 template __global__ void k_samples<5, 1>(KP, float*, float*);
 template __global__ void k_samples<5, 2>(KP, float*, float*);
+#define FVG_ZUPD(NV, M, T)                                                                                 \
+    template __global__ void k_zupdate<NV, M, UPD_RK2_2, T>(KP, const double*, const double*, double*); \
+    template __global__ void k_zupdate<NV, M, UPD_RK3_2, T>(KP, const double*, const double*, double*); \
+    template __global__ void k_zupdate<NV, M, UPD_RK3_3, T>(KP, const double*, const double*, double*);
+FVG_ZUPD(5, MODE_PLAIN, double)
+FVG_ZUPD(5, MODE_FIRST_ORDER, double)
+FVG_ZUPD(5, MODE_PLAIN, float)
+FVG_ZUPD(5, MODE_FIRST_ORDER, float)
+FVG_ZUPD(5, MODE_LEARNED, float)
+FVG_ZUPD(4, MODE_PLAIN, double)
+FVG_ZUPD(4, MODE_FIRST_ORDER, double)
+FVG_ZUPD(4, MODE_PLAIN, float)
+FVG_ZUPD(4, MODE_FIRST_ORDER, float)
+FVG_ZUPD(4, MODE_LEARNED, float)
+#undef FVG_ZUPD
 template __global__ void k_totals_partial<4>(const double*, long long, double*);
 template __global__ void k_totals_partial<5>(const double*, long long, double*);
 

@@ -26,7 +26,7 @@ _I = C.POINTER(C.c_int)
 EXPORTS = [
     "fvg_config_defaults", "fvg_config_set", "fvg_config_finalize", "fvg_dt_length", "fvg_create", "fvg_destroy",
     "fvg_last_error", "fvg_upload", "fvg_download", "fvg_residual", "fvg_step", "fvg_advance", "fvg_get_lagged_speed",
-    "fvg_set_lagged_speed", "fvg_conserved_totals", "fvg_kernel_times", "fvg_set_timing", "fvg_set_graphs", "fvg_riemann_batch",
+    "fvg_set_lagged_speed", "fvg_conserved_totals", "fvg_kernel_times", "fvg_set_timing", "fvg_set_graphs", "fvg_set_fused", "fvg_riemann_batch",
     "fvg_batch_create", "fvg_batch_destroy", "fvg_batch_last_error", "fvg_batch_upload", "fvg_batch_download",
     "fvg_batch_run", "fvg_batch_set_block", "fvg_batch_get_block", "fvg_batch_tune",
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
@@ -99,6 +99,7 @@ def lib():
         L.fvg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_char_p), _D, C.POINTER(C.c_longlong), C.c_int]
         L.fvg_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.fvg_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.fvg_set_fused.argtypes = [C.c_void_p, C.c_int]
         L.fvg_riemann_batch.argtypes = [P, C.c_int, _D, _D, _I, C.c_size_t, _D, _D]
         L.fvg_batch_create.argtypes = [P, C.c_int, _I, C.c_int, C.POINTER(C.c_void_p)]
         L.fvg_batch_destroy.argtypes = [C.c_void_p]
@@ -266,6 +267,10 @@ class Solver:
         d = np.empty(nf, dtype=np.float32)
         self._c(lib().fvg_indicator_samples(self._h, axis, X.ctypes.data_as(_F), d.ctypes.data_as(_F), nf))
         return X, d
+
+    def set_fused(self, on: bool):
+        """Runge-Kutta stages 2-3: the last axis' faces fused with the update (on, the default) or separate passes."""
+        self._c(lib().fvg_set_fused(self._h, int(on)))
 
     def set_graphs(self, on: bool):
         """advance() replays a captured HIP graph of steps (on, the default) or enqueues every kernel (off)."""

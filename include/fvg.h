@@ -158,6 +158,10 @@ int fvg_set_timing(fvg_ctx* ctx, int on);
  * context that holds four steps of the scheme; 0 enqueues every kernel. Results are the same bits either way;
  * with event timing on, kernels are always enqueued one by one. */
 int fvg_set_graphs(fvg_ctx* ctx, int on);
+/* Kernel sequence of the Runge-Kutta stages after the first (no counterpart in the reference): 1 (default)
+ * computes the faces of the last axis (z in 3-D, y in 2-D) and updates the cells in one pass; 0 keeps separate
+ * face and update passes. Same bits. */
+int fvg_set_fused(fvg_ctx* ctx, int on);
 
 /* Known-answer entry: the Riemann flux of n faces with host AoS conservative states (5 doubles per face),
  * axis per face, through the device kernel of the solver configured in cfg (riemann, shock_blend, EOS).

@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from cases import (VORTEX_DIVERGES_AT, VORTEX_HANCOCK_256, base_keys, golden_names, load_golden, matrix, smooth_state,
+from cases import (VORTEX_DIVERGES_AT, VORTEX_HANCOCK_256, base_keys, golden_names, load_golden, matrix, shock_state, smooth_state,
                    state_for, vortex_state)
 from fenice_amd import fvg
 from fenice_amd.config import Config, EOS_STIFFENED, HLLC, LOWMACH_HLLC
@@ -226,3 +226,37 @@ def test_graph_replays_match_eager_steps(oracle, case):
         a.residual()   # eager work between the replays
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("case", ["weno5-visc-rk3", "muscl-rk2-walls", "fp32-weno5-rk3", "2d-weno5-visc-rk3",
+                                  "2d-muscl-rk2-walls"])
+def test_fused_stages_match_separate_passes(oracle, case):
+    # the last axis' faces fused with the update (stages 2-3) give the bits of the separate face and update passes,
+    # and (FP64) the oracle's
+    say("fused stages", case)
+    n = (20, 18, 17) if not case.startswith("2d") else (70, 40, 1)
+    case = case.replace("2d-", "")
+    if case == "weno5-visc-rk3":
+        keys = base_keys(*n, reconstruction="weno5", use_viscous="true", mu="0.01", pr="0.71")
+        U = smooth_state(*n, 41, mach=0.3)
+    elif case == "muscl-rk2-walls":
+        keys = base_keys(*n, periodic=False, reconstruction="muscl", time_scheme="SSP Runge-Kutta 2", use_viscous="true",
+                         mu="0.01", bc_xmin="wall", bc_xmax="no_slip", bc_ymin="transmissive", bc_ymax="wall",
+                         bc_zmin="no_slip", bc_zmax="transmissive", boundary_order="2")
+        U = shock_state(*n, 42)
+    else:
+        keys = base_keys(*n, reconstruction="weno5", flux_precision="fp32", use_viscous="true", mu="0.01")
+        U = smooth_state(*n, 43, mach=0.3)
+    cfg = fvg.config_from_keys(keys)
+    out = []
+    for fused in (True, False):
+        with fvg.Solver(cfg) as s:
+            s.set_fused(fused)
+            s.upload(U)
+            t, k, _ = s.advance(0.0, 100.0, max_steps=4)
+            out.append((s.download(), t))
+    assert out[0][1] == out[1][1] and same_bits(out[0][0], out[1][0])
+    if case != "fp32-weno5-rk3":
+        Uo = U.copy()
+        oracle.run(cfg, Uo, 100.0, max_steps=4)
+        assert same_bits(out[0][0], Uo)

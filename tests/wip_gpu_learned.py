@@ -190,3 +190,55 @@ def test_audit_counts_and_fallback():
         s.advance(0.0, 100.0, max_steps=2)
         last, total, off, audits = s.indicator_stats()
         assert not off and audits == 2 and last[2] == 0 and last[3] == 0 and last[0] > 0
+
+
+def test_what_the_network_gains():
+    # train on the faces of two flows, evaluate on a third (unseen) flow: at the recall of the audit's limit the
+    # network sends fewer faces to WENO5-Z than the simplest sensor built from the same features (the largest
+    # normalised jump), and the learned run stays within FP32 accuracy of the FP32 WENO5-Z run
+    say("network gain")
+    from fenice_amd.workloads import kelvin_helmholtz, kh_keys, taylor_green, taylor_green_keys
+    X, y = [], []
+    for keys, U, steps in ((dict(kh_keys(192), flux_precision="fp32"), kelvin_helmholtz(192), (0, 120)),
+                           (dict(taylor_green_keys(48, "fp32")), taylor_green(48, mach=0.6), (0, 60))):
+        with fvg.Solver(fvg.config_from_keys(keys)) as s:
+            s.upload(U)
+            t, done = 0.0, 0
+            for k in steps:
+                if k > done:
+                    t, _, _ = s.advance(t, 1e9, max_steps=k - done)
+                    done = k
+                for axis in range(3 if int(keys["nz"]) > 1 else 2):
+                    Xa, d = s.samples(axis)
+                    ok = d >= 0
+                    X.append(Xa[ok, :15])
+                    y.append((d[ok] > 0.05).astype(np.float32))
+    X, y = np.concatenate(X), np.concatenate(y)
+    rng = np.random.default_rng(0)
+    keep = (y > 0.5) | (rng.random(y.size) < 0.2)
+    X, y = X[keep], y[keep]
+    p0 = (rng.normal(0, 1, L.NPARAM) * 0.25).astype(np.float32)
+    with fvg.Trainer(X.shape[0]) as tr:
+        tr.set_data(X, y)
+        p, loss = tr.fit(p0, epochs=12, batch=1024, lr=2e-3, pos_weight=float(min(20.0, 0.25 / max(y.mean(), 1e-6))), seed=3)
+    # unseen flow: a 2-D four-quadrant Riemann problem
+    from fenice_amd.train_indicator import _shock2d
+    keys = dict(gamma="1.4", use_viscous="false", riemann_solver="HLLC", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3",
+                cfl="0.5", flux_precision="fp32", nx="160", ny="160", nz="1", Lx="1.0", Ly="1.0", bc_xmin="transmissive",
+                bc_xmax="transmissive", bc_ymin="transmissive", bc_ymax="transmissive")
+    with fvg.Solver(fvg.config_from_keys(keys)) as s:
+        s.upload(_shock2d(160))
+        s.advance(0.0, 1e9, max_steps=80)
+        Xs, ds = zip(*(s.samples(a) for a in range(2)))
+    Xe = np.concatenate(Xs)
+    de = np.concatenate(ds)
+    ok = de >= 0
+    Xe, ye = Xe[ok, :15], de[ok] > 0.05
+    z = fvg.mlp_logits(p, Xe)
+    thr = np.sort(z[ye])[int(0.01 * ye.sum())]   # 99 % recall
+    skip_net = float((z[~ye] < thr).mean())
+    sensor = np.abs(Xe).max(axis=1)
+    sthr = np.sort(sensor[ye])[int(0.01 * ye.sum())]
+    skip_sensor = float((sensor[~ye] < sthr).mean())
+    say(f"smooth faces kept off WENO5-Z at 99% recall: network {skip_net:.3f}, max-jump sensor {skip_sensor:.3f}")
+    assert skip_net > skip_sensor + 0.05 and skip_net > 0.5
