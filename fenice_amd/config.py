@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("t_end", C.c_double), ("dt_max", C.c_double), ("max_steps", C.c_int),
         ("flux_precision", C.c_int),
         ("indicator", C.c_int), ("indicator_audit", C.c_int), ("indicator_label", C.c_double),
-        ("indicator_fn_limit", C.c_double),
+        ("indicator_fn_limit", C.c_double), ("indicator_fn_floor", C.c_double),
     ]
 
     @staticmethod
@@ -58,6 +58,7 @@ class Config(C.Structure):
         c.t_end, c.dt_max, c.max_steps = 0.005, 0.0, 0
         c.flux_precision = FLUX_FP64
         c.indicator, c.indicator_audit, c.indicator_label, c.indicator_fn_limit = INDICATOR_NONE, 50, 0.05, 0.01
+        c.indicator_fn_floor = 1e-4
         return c
 
     @staticmethod

@@ -41,8 +41,9 @@ constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along 
 constexpr int kTotalsBlocks = 1024;
 thread_local std::string g_create_error;
 
-enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_N };
-const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_zupdate"};
+enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_AUDIT, FAM_N };
+const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_zupdate",
+                                   "k_audit"};
 
 struct Err {
     int code;
@@ -80,7 +81,7 @@ struct fvg_ctx {
     MlpDev* mlp = nullptr;
     IndState* ind = nullptr;
     bool graphs = true;                // fvg_advance replays a captured graph of steps
-    bool fused = true;                 // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
+    bool fused = false;                // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
     hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
 };
 
@@ -158,16 +159,16 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
         return rc;
     if constexpr (std::is_same<T, float>::value) {
         if (mode == MODE_LEARNED) {
-#define LFACES(KERNEL, D) \
-    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+#define LFACES(KERNEL, FAM, D) \
+    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
             if (audit) {
-                if ((rc = LFACES(k_audit, 0)) || (rc = LFACES(k_audit, 1))) return rc;
+                if ((rc = LFACES(k_audit, FAM_AUDIT, 0)) || (rc = LFACES(k_audit, FAM_AUDIT, 1))) return rc;
                 if constexpr (NV == 5)
-                    if ((rc = LFACES(k_audit, 2))) return rc;
+                    if ((rc = LFACES(k_audit, FAM_AUDIT, 2))) return rc;
             }
-            if ((rc = LFACES(k_faces_learned, 0)) || (rc = LFACES(k_faces_learned, 1))) return rc;
+            if ((rc = LFACES(k_faces_learned, FAM_FACES_X, 0)) || (rc = LFACES(k_faces_learned, FAM_FACES_Y, 1))) return rc;
             if constexpr (NV == 5)
-                if ((rc = LFACES(k_faces_learned, 2))) return rc;
+                if ((rc = LFACES(k_faces_learned, FAM_FACES_Z, 2))) return rc;
 #undef LFACES
             return FVG_OK;
         }
@@ -464,30 +465,36 @@ static unsigned bf16_host(float x) {
 }
 
 // The network's parameter vector (W1 32x15, b1, W2 32x32, b2, w3, b3) as the lanes of a wavefront read it
-// (fvg_learned.hpp): b1 folded into column 15 of W1, the weight matrices as bfloat16 MFMA fragments.
+// (fvg_learned.hpp): b1 folded into column 15 of W1, the weight matrices as bfloat16 MFMA fragments, W2's columns
+// in the order the layer-1 accumulator hands its rows to layer 2, b2 and w3 at each lane half's rows.
 MlpDev pack_mlp(const float* P, float thr) {
     MlpDev m{};
     const float* W1 = P;
     const float* b1 = P + 480;
     const float* W2 = P + 512;
     auto w1 = [&](int h, int k) { return k < 15 ? W1[h * 15 + k] : b1[h]; };
+    auto frag = [](const float* v) {
+        Frag8 f{0, 0};
+        for (int s = 0; s < 4; ++s) f.lo |= (unsigned long long)bf16_host(v[s]) << (16 * s);
+        for (int s = 0; s < 4; ++s) f.hi |= (unsigned long long)bf16_host(v[4 + s]) << (16 * s);
+        return f;
+    };
     for (int l = 0; l < 64; ++l) {
-        const int row = l % 32, k0 = 4 * (l / 32);
-        for (int kc = 0; kc < 2; ++kc) {
-            unsigned long long v = 0;
-            for (int i = 0; i < 4; ++i) v |= (unsigned long long)bf16_host(w1(row, 8 * kc + k0 + i)) << (16 * i);
-            m.w1[kc][l] = v;
-        }
-        for (int kc = 0; kc < 4; ++kc) {
-            unsigned long long v = 0;
-            for (int i = 0; i < 4; ++i) v |= (unsigned long long)bf16_host(W2[row * 32 + 8 * kc + k0 + i]) << (16 * i);
-            m.w2[kc][l] = v;
+        const int row = l % 32, h = l / 32;
+        float v[8];
+        for (int s = 0; s < 8; ++s) v[s] = w1(row, 8 * h + s);
+        m.w1[l] = frag(v);
+        for (int c = 0; c < 2; ++c) {
+            for (int s = 0; s < 8; ++s) v[s] = W2[row * 32 + 16 * c + 8 * (s / 4) + 4 * h + s % 4];
+            m.w2[c][l] = frag(v);
         }
     }
-    for (int h = 0; h < 32; ++h) {
-        m.b2[h] = P[1536 + h];
-        m.w3[h] = P[1568 + h];
-    }
+    for (int h = 0; h < 2; ++h)
+        for (int e = 0; e < 16; ++e) {
+            const int r = 8 * (e / 4) + 4 * h + e % 4;
+            m.b2h[h][e] = P[1536 + r];
+            m.w3h[h][e] = P[1568 + r];
+        }
     m.b3 = P[1600];
     m.thr = thr;
     return m;
@@ -661,6 +668,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     p.ind_audit = cfg.indicator_audit;
     p.ind_label = (float)cfg.indicator_label;
     p.ind_fn_limit = cfg.indicator_fn_limit;
+    p.ind_fn_floor = cfg.indicator_fn_floor;
     DevState s = fresh_state(c);
     if ((rc = write_state(c, s)) || (rc = (hipStreamSynchronize(c->stream) == hipSuccess ? 0 : FVG_EHIP))) {
         const std::string m = c->err;

@@ -86,6 +86,7 @@ void fvg_config_defaults(fvg_config* c) {
     c->indicator_audit = 50;
     c->indicator_label = 0.05;
     c->indicator_fn_limit = 0.01;
+    c->indicator_fn_floor = 1e-4;
 }
 
 int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* err, size_t n) {
@@ -176,6 +177,7 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
     if (key == "indicator_audit") return integer(c->indicator_audit);
     if (key == "indicator_label") return num(c->indicator_label);
     if (key == "indicator_fn_limit") return num(c->indicator_fn_limit);
+    if (key == "indicator_fn_floor") return num(c->indicator_fn_floor);
     static const char* bc_keys[6] = {"bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax", "bc_zmin", "bc_zmax"};
     for (int s = 0; s < 6; ++s) {
         if (key != bc_keys[s]) continue;
@@ -218,8 +220,8 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
     if (c->indicator == FVG_INDICATOR_LEARNED &&
         (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))
         return fail(err, n, "indicator = learned needs reconstruction = weno5, flux_precision = fp32 and a Runge-Kutta scheme");
-    if (c->indicator_audit < 0 || !(c->indicator_label > 0.0) || !(c->indicator_fn_limit >= 0.0))
-        return fail(err, n, "indicator_audit must be >= 0, indicator_label > 0 and indicator_fn_limit >= 0");
+    if (c->indicator_audit < 0 || !(c->indicator_label > 0.0) || !(c->indicator_fn_limit >= 0.0) || !(c->indicator_fn_floor >= 0.0))
+        return fail(err, n, "indicator_audit must be >= 0, indicator_label > 0 and indicator_fn_limit, indicator_fn_floor >= 0");
     // a first-order scheme has no reconstruction to extend to the boundary faces (Params.cpp:460)
     if (c->boundary_order == 2 && c->reconstruction == FVG_RECON_NONE) c->boundary_order = 1;
     return FVG_OK;

@@ -1159,8 +1159,9 @@ __global__ void k_indicator_control(KP p, int op) {
     if (audit_step) {
         for (int q = 0; q < 4; ++q) d->total[q] += d->last[q];
         d->audits += 1;
-        const double troubled = double(d->last[0] + d->last[2]);
-        if (troubled > 0.0 && double(d->last[2]) > p.ind_fn_limit * troubled) d->off = 1;
+        const double troubled = double(d->last[0] + d->last[2]), missed = double(d->last[2]);
+        const double faces = double(d->last[0] + d->last[1] + d->last[2] + d->last[3]);
+        if (troubled > 0.0 && missed > p.ind_fn_limit * troubled && missed > p.ind_fn_floor * faces) d->off = 1;
     }
     d->steps += 1;
 }

@@ -58,6 +58,7 @@ struct KP {
     int ind_audit;            // steps between audits (0: none)
     float ind_label;          // weight departure of a troubled face
     double ind_fn_limit;      // missed fraction of troubled faces that switches the indicator off
+    double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };
 
 enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3 };

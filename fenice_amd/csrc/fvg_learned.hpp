@@ -2,14 +2,17 @@
 // weights of weno5z, Schemes.cpp:338-353).
 //
 // Features of a face (15): for density, pressure and the velocity component with the largest total variation over
-// the face's six-cell stencil c0..c5, the five differences c_{j+1} - c_j divided by
-// sqrt(sum of their squares + eps_rel * mean(c^2) + 1e-30) (eps_rel: WENO's relative regularisation).
+// the face's six-cell stencil c0..c5, the four second differences and the largest first difference, divided by
+// sqrt(sum of the squared first differences + eps_rel * mean(c^2) + 1e-30) (eps_rel: WENO's relative
+// regularisation). Smooth data gives small second differences; a jump or grid-scale noise gives O(1) ones.
 // Network: 15 -> 32 (ReLU) -> 32 (ReLU) -> 1 logit; a face is troubled when the logit exceeds the threshold.
 // Inference runs on the matrix cores, one wavefront of 64 faces at a time: two column blocks of 32 faces, each
-// through v_mfma_f32_32x32x8_bf16 (hidden units as rows, faces as columns). With that orientation the accumulator
-// layout of layer 1 (lane l holds rows 8j + 4(l/32) + i, i < 4, of face column l % 32) is the B-operand layout of
-// layer 2's K chunk j, so the hidden layer never leaves the registers. x, W1 (bias b1 folded into column 15 against
-// x15 = 1) and the first hidden layer are bfloat16 (round to nearest even); accumulation, b2, w3 and b3 are float.
+// through v_mfma_f32_32x32x16_bf16 (hidden units as rows, faces as columns): one product for layer 1 (K = 16),
+// two for layer 2 (K = 32). Lane l of a 32x32 accumulator holds rows 8j + 4h + i (h = l/32, j < 4, i < 4) of face
+// column l % 32; its elements 8c .. 8c+7 are layer 2's B operand for K chunk c once W2's columns are permuted to
+// match (K index 16c + 8h + s <- hidden row 16c + 8(s/4) + 4h + s%4), so the hidden layer never leaves the
+// registers. x, W1 (bias b1 folded into column 15 against x15 = 1) and the first hidden layer are bfloat16 (round
+// to nearest even); accumulation, b2, w3 and b3 are float.
 #pragma once
 
 #include "fvg_device.hpp"
@@ -20,10 +23,14 @@ constexpr int MLP_NIN = 15, MLP_NH = 32;
 constexpr int MLP_NPARAM = MLP_NH * MLP_NIN + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1;   // 1601
 
 // The network as the lanes of a wavefront read it.
+struct Frag8 {
+    unsigned long long lo, hi;   // eight bfloat16 values
+};
 struct MlpDev {
-    unsigned long long w1[2][64];   // W1 fragment of K chunk kc for lane l: row l % 32, columns 8kc + 4(l/32) + 0..3
-    unsigned long long w2[4][64];   // W2 fragments, K chunks 0..3
-    float b2[32], w3[32];
+    Frag8 w1[64];        // W1 (b1 in column 15) for lane l: row l % 32, columns 8h .. 8h+7 (h = l / 32)
+    Frag8 w2[2][64];     // W2 for K chunk c, lane l: row l % 32, hidden rows 16c + 8(s/4) + 4h + s%4, s < 8
+    float b2h[2][16];    // b2 at the accumulator rows of lane half h: 8(e/4) + 4h + e%4
+    float w3h[2][16];    // w3 at the same rows
     float b3, thr;
 };
 
@@ -37,7 +44,7 @@ struct IndState {
     int pending;                   // the current step is being taken
 };
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // float -> bfloat16 bits, round to nearest even (finite inputs)
@@ -49,7 +56,10 @@ __device__ __forceinline__ unsigned long long pack4(float a, float b, float c, f
     return (unsigned long long)bf16_bits(a) | ((unsigned long long)bf16_bits(b) << 16) | ((unsigned long long)bf16_bits(c) << 32) |
            ((unsigned long long)bf16_bits(d) << 48);
 }
-__device__ __forceinline__ bf16x4 as_frag(unsigned long long v) { return __builtin_bit_cast(bf16x4, v); }
+__device__ __forceinline__ bf16x8 as_frag(unsigned long long lo, unsigned long long hi) {
+    Frag8 f{lo, hi};
+    return __builtin_bit_cast(bf16x8, f);
+}
 __device__ __forceinline__ unsigned long long xor32(unsigned long long v) {
     const int lo = __shfl_xor((int)(unsigned)v, 32, 64);
     const int hi = __shfl_xor((int)(unsigned)(v >> 32), 32, 64);
@@ -88,23 +98,24 @@ template <int NV, typename T> __device__ __forceinline__ T face_departure(const 
     return d;
 }
 
-// five normalised differences of one variable over the six-cell stencil
-__device__ __forceinline__ void diff5(float v0, float v1, float v2, float v3, float v4, float v5, float eps_rel, float* x) {
+// five features of one variable over the six-cell stencil: its four second differences and its largest first
+// difference, divided by N = sqrt(sum of the squared first differences + eps_rel * mean(c^2) + 1e-30)
+__device__ __forceinline__ void feat5(float v0, float v1, float v2, float v3, float v4, float v5, float eps_rel, float* x) {
     const float d0 = v1 - v0, d1 = v2 - v1, d2 = v3 - v2, d3 = v4 - v3, d4 = v5 - v4;
     const float s = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3 + d4 * d4;
     const float m = (v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3 + v4 * v4 + v5 * v5) * (1.0f / 6.0f);
     const float r = __builtin_amdgcn_rsqf(s + eps_rel * m + 1e-30f);
-    x[0] = d0 * r;
-    x[1] = d1 * r;
-    x[2] = d2 * r;
-    x[3] = d3 * r;
-    x[4] = d4 * r;
+    x[0] = (d1 - d0) * r;
+    x[1] = (d2 - d1) * r;
+    x[2] = (d3 - d2) * r;
+    x[3] = (d4 - d3) * r;
+    x[4] = fmaxf(fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fmaxf(fabsf(d2), fabsf(d3))), fabsf(d4)) * r;
 }
 
 // the 15 features of a face (x[15] = 1 carries the first layer's bias)
 template <int NV> __device__ __forceinline__ void face_features(const float (*c)[NV], float eps_rel, float* x) {
-    diff5(c[0][0], c[1][0], c[2][0], c[3][0], c[4][0], c[5][0], eps_rel, x);
-    diff5(c[0][3], c[1][3], c[2][3], c[3][3], c[4][3], c[5][3], eps_rel, x + 5);
+    feat5(c[0][0], c[1][0], c[2][0], c[3][0], c[4][0], c[5][0], eps_rel, x);
+    feat5(c[0][3], c[1][3], c[2][3], c[3][3], c[4][3], c[5][3], eps_rel, x + 5);
     // the velocity slot with the largest total variation (the first on ties), picked by selects
     float v[6] = {c[0][1], c[1][1], c[2][1], c[3][1], c[4][1], c[5][1]}, best = -1.0f;
 #pragma unroll
@@ -118,42 +129,46 @@ template <int NV> __device__ __forceinline__ void face_features(const float (*c)
 #pragma unroll
         for (int j = 0; j < 6; ++j) v[j] = take ? c[j][q] : v[j];
     }
-    diff5(v[0], v[1], v[2], v[3], v[4], v[5], eps_rel, x + 10);
+    feat5(v[0], v[1], v[2], v[3], v[4], v[5], eps_rel, x + 10);
     x[15] = 1.0f;
 }
 
 // Logit of this lane's face; every lane of the wavefront must call it (the matrix instructions use all 64).
 __device__ __forceinline__ float mlp_logit_wave(const MlpDev* __restrict__ m, const float* x) {
     const int lane = threadIdx.x & 63, half = lane >> 5;
-    unsigned long long g[4], o[4];
+    unsigned long long g[4], o[4];   // features 4q .. 4q+3 of the own face and of the partner lane's (l ^ 32)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         g[q] = pack4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
         o[q] = xor32(g[q]);
     }
+    const Frag8 w1 = m->w1[lane], w2a = m->w2[0][lane], w2b = m->w2[1][lane];
     float out = 0.0f;
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
-        // B operand (features of face 32 cb + l % 32; K rows 8kc + 4 half + 0..3): own or the partner lane's
-        const unsigned long long b0 = cb == 0 ? (half ? o[1] : g[0]) : (half ? g[1] : o[0]);
-        const unsigned long long b1 = cb == 0 ? (half ? o[3] : g[2]) : (half ? g[3] : o[2]);
+        // B operand: features 8h .. 8h+7 of face 32 cb + l % 32, owned by this lane when cb == h, else by the partner
+        const bool own = cb == half;
+        const unsigned long long b0 = own ? (half ? g[2] : g[0]) : (half ? o[2] : o[0]);
+        const unsigned long long b1 = own ? (half ? g[3] : g[1]) : (half ? o[3] : o[1]);
         f32x16 acc = {};
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w1[0][lane]), as_frag(b0), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w1[1][lane]), as_frag(b1), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_frag(w1.lo, w1.hi), as_frag(b0, b1), acc, 0, 0, 0);
         f32x16 acc2;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) acc2[e] = m->b2[8 * (e >> 2) + 4 * half + (e & 3)];
+        for (int e = 0; e < 16; ++e) acc2[e] = m->b2h[half][e];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const unsigned long long h = pack4(fmaxf(acc[4 * j], 0.0f), fmaxf(acc[4 * j + 1], 0.0f), fmaxf(acc[4 * j + 2], 0.0f),
-                                               fmaxf(acc[4 * j + 3], 0.0f));
-            acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(as_frag(m->w2[j][lane]), as_frag(h), acc2, 0, 0, 0);
+        for (int c = 0; c < 2; ++c) {
+            const unsigned long long h0 = pack4(fmaxf(acc[8 * c], 0.0f), fmaxf(acc[8 * c + 1], 0.0f), fmaxf(acc[8 * c + 2], 0.0f),
+                                                fmaxf(acc[8 * c + 3], 0.0f));
+            const unsigned long long h1 = pack4(fmaxf(acc[8 * c + 4], 0.0f), fmaxf(acc[8 * c + 5], 0.0f), fmaxf(acc[8 * c + 6], 0.0f),
+                                                fmaxf(acc[8 * c + 7], 0.0f));
+            const Frag8 w = c == 0 ? w2a : w2b;
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_frag(w.lo, w.hi), as_frag(h0, h1), acc2, 0, 0, 0);
         }
         float part = 0.0f;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) part += m->w3[8 * (e >> 2) + 4 * half + (e & 3)] * fmaxf(acc2[e], 0.0f);
+        for (int e = 0; e < 16; ++e) part += m->w3h[half][e] * fmaxf(acc2[e], 0.0f);
         const float z = part + __shfl_xor(part, 32, 64) + m->b3;
-        if (cb == half) out = z;
+        if (own) out = z;
     }
     return out;
 }

@@ -269,7 +269,8 @@ class Solver:
         return X, d
 
     def set_fused(self, on: bool):
-        """Runge-Kutta stages 2-3: the last axis' faces fused with the update (on, the default) or separate passes."""
+        """Runge-Kutta stages 2-3: the last axis' faces fused with the update (on) or separate passes (off, the
+        default). Same bits."""
         self._c(lib().fvg_set_fused(self._h, int(on)))
 
     def set_graphs(self, on: bool):

@@ -62,8 +62,9 @@ enum fvg_flux_precision { FVG_FLUX_FP64 = 0, FVG_FLUX_FP32 = 1 };
  * and reconstruction = weno5): a small network classifies every face from 15 features of its stencil; faces it
  * calls smooth take the linear upwind-5 value instead of WENO5-Z. An audit every indicator_audit steps compares its
  * decisions with the WENO5-Z weight departure (label: departure > indicator_label) and switches the indicator off
- * for the rest of the context's life when the missed troubled faces exceed indicator_fn_limit of the troubled
- * ones. */
+ * for the rest of the context's life when the missed troubled faces exceed both indicator_fn_limit of the troubled
+ * faces and indicator_fn_floor of all the audited faces (the second keeps a handful of isolated label flips in a
+ * smooth flow, e.g. where a velocity component crosses zero, from switching it off). */
 enum fvg_indicator { FVG_INDICATOR_NONE = 0, FVG_INDICATOR_LEARNED = 1 };
 
 /* The keys of the path (cpp/src/Params.cpp:98-223 and SimParams::from), typed. Defaults: fvg_config_defaults. */
@@ -91,6 +92,7 @@ typedef struct fvg_config {
     int indicator_audit;         /* steps between audits (default 50; 0: no audit) */
     double indicator_label;      /* weight departure above which a face is troubled (default 0.05) */
     double indicator_fn_limit;   /* fallback limit of the missed fraction of troubled faces (default 0.01) */
+    double indicator_fn_floor;   /* fallback limit of the missed fraction of all audited faces (default 1e-4) */
 } fvg_config;
 
 /* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,
@@ -158,9 +160,9 @@ int fvg_set_timing(fvg_ctx* ctx, int on);
  * context that holds four steps of the scheme; 0 enqueues every kernel. Results are the same bits either way;
  * with event timing on, kernels are always enqueued one by one. */
 int fvg_set_graphs(fvg_ctx* ctx, int on);
-/* Kernel sequence of the Runge-Kutta stages after the first (no counterpart in the reference): 1 (default)
- * computes the faces of the last axis (z in 3-D, y in 2-D) and updates the cells in one pass; 0 keeps separate
- * face and update passes. Same bits. */
+/* Kernel sequence of the Runge-Kutta stages after the first (no counterpart in the reference): 1 computes the
+ * faces of the last axis (z in 3-D, y in 2-D) and updates the cells in one pass (1024-thread tiles); 0 (default)
+ * keeps separate face and update passes. Same bits. */
 int fvg_set_fused(fvg_ctx* ctx, int on);
 
 /* Known-answer entry: the Riemann flux of n faces with host AoS conservative states (5 doubles per face),

@@ -10,9 +10,10 @@ The indicator has no reference counterpart beyond the functions its labels come 
 - smooth faces take the linear upwind-5 value (2c0 - 13c1 + 47c2 + 27c3 - 3c4) / 60, which is weno5z with a = d
   (exact for quartic cell-average data);
 - features of a face (15): for density, pressure and the velocity component with the largest total variation over
-  the face's six-cell stencil c0..c5 (cells t-3 .. t+2 of a face between cells t-1 and t), the five differences
-  c_{j+1} - c_j divided by sqrt(sum of their squares + eps_rel * mean(c^2) + 1e-30), eps_rel the WENO
-  regularisation (the same relative scaling WENO5-Z applies);
+  the face's six-cell stencil c0..c5 (cells t-3 .. t+2 of a face between cells t-1 and t), the four second
+  differences d_{j+1} - d_j and the largest |d_j| of its first differences d_j = c_{j+1} - c_j, divided by
+  sqrt(sum of d_j^2 + eps_rel * mean(c^2) + 1e-30), eps_rel the WENO regularisation (the same relative scaling
+  WENO5-Z applies);
 - network: 15 -> 32 (ReLU) -> 32 (ReLU) -> 1 logit; troubled when logit > threshold. Inference rounds x, W1, b1, W2
   and the first hidden layer to bfloat16 (round to nearest even) and accumulates in float32.
 Parameter vector layout (float32, 1601 values): W1 (32 x 15, row-major), b1 (32), W2 (32 x 32), b2 (32), w3 (32),
@@ -96,7 +97,8 @@ def features(st, eps_rel, nvel):
     for c in (st[:, 0, :], st[:, 3, :], v):
         d = np.diff(c, axis=0)
         norm = np.sqrt((d * d).sum(axis=0) + eps_rel * (c * c).mean(axis=0) + 1e-30)
-        out.append((d / norm).T)
+        out.append((np.diff(d, axis=0) / norm).T)
+        out.append((np.abs(d).max(axis=0) / norm)[:, None])
     return np.concatenate(out, axis=1)
 
 

@@ -46,13 +46,17 @@ def test_feature_invariances():
     rng = np.random.default_rng(1)
     st = 1.0 + 0.1 * rng.random((6, 5, 50))
     f = L.features(st, 1e-6, 3)
-    assert f.shape == (50, 15) and np.all(np.abs(f) <= 1.0)
+    assert f.shape == (50, 15) and np.all(np.abs(f) <= np.sqrt(2.0) + 1e-12)   # |d1 - d0| <= sqrt(2) N
     st2 = st.copy()
     st2[:, 0] *= 7.0   # density scale: its five features do not change
     assert np.allclose(L.features(st2, 1e-6, 3)[:, :5], f[:, :5], atol=1e-12)
     ramp = (0.5 + 0.01 * np.arange(6.0))[:, None, None] * np.ones((6, 5, 1))
     fr = L.features(ramp, 0.0, 3)
-    assert np.allclose(fr[0, :5], 1.0 / np.sqrt(5.0))   # a linear ramp: five equal differences
+    assert np.allclose(fr[0, :4], 0.0, atol=1e-12)           # a linear ramp: no second differences
+    assert np.isclose(fr[0, 4], 1.0 / np.sqrt(5.0))          # five equal first differences
+    step = np.where(np.arange(6) < 3, 1.0, 0.125)[:, None, None] * np.ones((6, 5, 1))
+    fs = L.features(step, 0.0, 3)
+    assert np.isclose(fs[0, 4], 1.0) and np.isclose(fs[0, 1], -1.0) and np.isclose(fs[0, 2], 1.0)   # one jump
 
 
 def test_bf16_rounding():
