@@ -100,6 +100,9 @@ int set_err(fvg_ctx* c, int code, const std::string& m) {
     } while (0)
 
 unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// elements of one padded row of the primitive planes: the interior from PX0, PH halo cells on either side, rounded
+// up to a multiple of PXA
+int padded_row(int nx) { return (PX0 + nx + PH + PXA - 1) / PXA * PXA; }
 
 hipEvent_t take_event(fvg_ctx* c) {
     hipEvent_t e;
@@ -556,6 +559,11 @@ void fill_kernel_params(const fvg_config& cfg, KP& p) {
                 if (bt != FVG_BC_PERIODIC && bt != FVG_BC_NONREFLECTING && n3[a] >= p.Dh) p.halo[2 * a + hi] = 2;
             }
     }
+    // padded primitive planes
+    p.px = padded_row(cfg.nx);
+    p.py = cfg.ny + 2 * PH;
+    p.phz = three ? PH : 0;
+    p.npad = (long long)p.px * p.py * (cfg.nz + 2 * p.phz);
     p.cfl = cfg.cfl;
     p.Ldt = fvg_dt_length(&cfg);
     p.viscous = visc;
@@ -619,7 +627,8 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     rc = rc ? rc : checked_alloc(c, &c->Rb, (size_t)NV * c->nc);
     c->fp32 = cfg.flux_precision == FVG_FLUX_FP32;
     const size_t fe = c->fp32 ? sizeof(float) : sizeof(double);
-    rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * c->nc, fe);
+    const long long npad = (long long)padded_row(cfg.nx) * (cfg.ny + 2 * PH) * (cfg.nz + (three ? 2 * PH : 0));
+    rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * npad, fe);
     if (visc) rc = rc ? rc : checked_alloc(c, &c->G, (size_t)ND * ND * c->nc, fe);
     if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
     for (int d = 0; d < ND; ++d) {

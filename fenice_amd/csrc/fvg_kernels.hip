@@ -36,10 +36,18 @@ template <int NV> __device__ __forceinline__ void load_cons(const KP& p, long lo
 #pragma unroll
     for (int q = 0; q < NV; ++q) u[q] = p.U[q * p.nc + c];
 }
-template <int NV, typename T = double> __device__ __forceinline__ void load_prim(const KP& p, long long c, T* w) {
+// element of cell (i, j, k) in the padded primitive planes (each coordinate from -PH)
+__device__ __forceinline__ long long pidx(const KP& p, int i, int j, int k) {
+    return ((long long)(k + p.phz) * p.py + (j + PH)) * p.px + (i + PX0);
+}
+__device__ __forceinline__ long long pidx_d(const KP& p, int d, int i, int j, int k, int t) {
+    return pidx(p, d == 0 ? t : i, d == 1 ? t : j, d == 2 ? t : k);
+}
+// the primitives of padded element pc
+template <int NV, typename T = double> __device__ __forceinline__ void load_prim(const KP& p, long long pc, T* w) {
     const T* W = (const T*)p.W;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) w[q] = W[q * p.nc + c];
+    for (int q = 0; q < NV; ++q) w[q] = W[q * p.npad + pc];
 }
 
 template <int NV, typename T> __device__ __forceinline__ void to_t(const double* a, T* b) {
@@ -83,28 +91,11 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 // The padded primitive state of the reconstruction at the cell whose coordinate along d is t (the other two
 // from (i, j, k)): the cell itself, or outside the domain the periodic wrap, the ghost-slab layer
 // (fill_ghost_slab, Boundaries.cpp:202-228) or the edge cell (pad_primitives, Schemes.cpp:915-969).
+// The primitives a reconstruction stencil along d takes at coordinate t (-PH <= t < n + PH) of the line through
+// (i, j, k): the cell itself inside the mesh, else the halo value k_cells stored there (push_halo).
 template <int NV, typename T = double>
 __device__ __forceinline__ void stencil_prim(const KP& p, int d, int i, int j, int k, int t, T* w) {
-    const int n = ncell(p, d);
-    if (t >= 0 && t < n) {
-        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, t), w);
-        return;
-    }
-    const int hi = t >= n, side = 2 * d + hi;
-    const int h = halo_of(p, side);
-    if (h == 1) {
-        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, hi ? t - n : t + n), w);
-    } else if (h == 2) {
-        const int l = hi ? t - n : -1 - t;
-        double ui[NV], ug[NV], wd[NV];
-        load_cons<NV>(p, cidx_d(p, d, i, j, k, hi ? n - 1 - l : l), ui);
-        ghost_state<NV>(p.rs.eos, p.bp, side, ui, ug);
-        c2p<NV>(p.rs.eos, ug, wd, (double*)nullptr);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
-    } else {
-        load_prim<NV, T>(p, cidx_d(p, d, i, j, k, t < 0 ? 0 : n - 1), w);
-    }
+    load_prim<NV, T>(p, pidx_d(p, d, i, j, k, t), w);
 }
 
 // First-order state outside side `side` (along axis d) of the boundary cell (i, j, k) (apply_side,
@@ -227,10 +218,7 @@ __device__ __forceinline__ void visc_pad_prim(const KP& p, int i, int j, int k, 
 // cell's viscous speed bound (Physics.cpp:44-76 2-D, 520-555 and 733-741 3-D; ViscousBound.hpp:16-24), FP64 in the
 // reference's operation order.
 template <int NV>
-__device__ __forceinline__ double cell_gradients_f64(const KP& p, long long c, const double* w) {
-    const int i = (int)(c % p.nx);
-    const long long r = c / p.nx;
-    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+__device__ __forceinline__ double cell_gradients_f64(const KP& p, long long c, int i, int j, int k, const double* w) {
     const double rho = w[0];
     constexpr int ND = NV - 2;
     double grad[ND][ND + 1];   // grad[b][slot]: slots 0 (rho) and the momenta 1..ND
@@ -262,10 +250,7 @@ __device__ __forceinline__ double cell_gradients_f64(const KP& p, long long c, c
 
 // The same in float for the FP32-flux mode: products with the reciprocals of 2h and rho in place of divisions.
 template <int NV>
-__device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, const float* w) {
-    const int i = (int)(c % p.nx);
-    const long long r = c / p.nx;
-    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+__device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, int i, int j, int k, const float* w) {
     const float rho = w[0], inv_rho = __builtin_amdgcn_rcpf(rho);
     constexpr int ND = NV - 2;
     float grad[ND][ND + 1];
@@ -296,11 +281,58 @@ __device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, c
     return double(mv) + double(sound_speed(p.rs.eos, w[3], rho)) + 2.0 * p.Ldt * p.inv_h2 * nu;
 }
 
+// The halo of the padded primitive planes, pushed by the cells it copies (the padded state of Simulation.cpp:863-905,
+// 1109-1170): along an axis with a periodic halo a cell within PH of one side writes its primitives into the halo
+// beyond the other side; with a ghost slab (boundary_order = 2) it writes the primitives of its ghost state at its
+// mirror position; a side with neither has its halo layers copy the edge cell. u: the cell's state, w: its
+// primitives as stored.
+template <int NV, typename T>
+__device__ __forceinline__ void push_halo(const KP& p, int i, int j, int k, const double* u, const T* w) {
+    T* Wo = (T*)p.W;
+    auto put = [&](int d, int t, const T* v) {
+        const long long pc = pidx_d(p, d, i, j, k, t);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = v[q];
+    };
+    auto put_ghost = [&](int d, int side, int t) {
+        double ug[NV], wg[NV];
+        T v[NV];
+        ghost_state<NV>(p.rs.eos, p.bp, side, u, ug);
+        c2p<NV>(p.rs.eos, ug, wg, (double*)nullptr);
+        to_t<NV>(wg, v);
+        put(d, t, v);
+    };
+#pragma unroll
+    for (int d = 0; d < NV - 2; ++d) {
+        const int t = coord(d, i, j, k), n = ncell(p, d);
+        if (t >= PH && t < n - PH) continue;   // no halo value comes from this cell along d
+        const int hl = halo_of(p, 2 * d), hh = halo_of(p, 2 * d + 1);
+        if (hl == 1) {   // periodic: both sides wrap
+            if (t >= n - PH) put(d, t - n, w);
+            if (t < PH) put(d, t + n, w);
+            continue;
+        }
+        if (hl == 2) {
+            if (t < PH) put_ghost(d, 2 * d, -1 - t);
+        } else if (t == 0) {
+            for (int l = 1; l <= PH; ++l) put(d, -l, w);
+        }
+        if (hh == 2) {
+            if (t >= n - PH) put_ghost(d, 2 * d + 1, 2 * n - 1 - t);
+        } else if (t == n - 1) {
+            for (int l = 0; l < PH; ++l) put(d, n + l, w);
+        }
+    }
+}
 template <int NV, typename T = double>
 __global__ __launch_bounds__(256) void k_cells(KP p) {
     const long long c = xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
     if (c < p.nc && !p.st->done) {
+        const int i = (int)(c % p.nx);
+        const long long r = c / p.nx;
+        const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+        const long long pc = pidx(p, i, j, k);
         if constexpr (std::is_same<T, float>::value) {
             // FP32 fluxes: the primitives are the FP64 ones rounded to float (the viscous ghosts of a periodic side
             // round the same FP64 primitives, so the two faces of a periodic seam stay identical); the viscous cell
@@ -312,16 +344,18 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             to_t<NV>(wd, w);
             float* Wo = (float*)p.W;
 #pragma unroll
-            for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = w[q];
-            if (p.viscous) local = smax(local, cell_gradients_f32<NV>(p, c, w));
+            for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
+            push_halo<NV, float>(p, i, j, k, u, w);
+            if (p.viscous) local = smax(local, cell_gradients_f32<NV>(p, c, i, j, k, w));
         } else {
             double u[NV], w[NV];
             load_cons<NV>(p, c, u);
             c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
             double* Wo = (double*)p.W;
 #pragma unroll
-            for (int q = 0; q < NV; ++q) Wo[q * p.nc + c] = w[q];
-            if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, w));
+            for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
+            push_halo<NV, double>(p, i, j, k, u, w);
+            if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, i, j, k, w));
         }
     }
     if (p.viscous) block_max_to(local, &p.st->s_visc);
@@ -354,7 +388,7 @@ __global__ __launch_bounds__(256) void k_hancock(KP p) {
         const int j = (int)(r % p.ny), k = (int)(r / p.ny);
         double st[NS][NV];
         double w0[NV];
-        load_prim<NV, double>(p, c, w0);
+        load_prim<NV, double>(p, pidx(p, i, j, k), w0);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const int t = coord(d, i, j, k), n = ncell(p, d);
@@ -455,7 +489,7 @@ __device__ __forceinline__ void viscous_face(const KP& p, int i, int j, int k, i
 #pragma unroll
             for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
         } else {
-            load_prim<NV, T>(p, cL, w);
+            load_prim<NV, T>(p, pidx_d(p, D, i, j, k, t - 1), w);
         }
 #pragma unroll
         for (int a = 0; a < ND; ++a) vL[a] = w[mom(a)];
@@ -466,7 +500,7 @@ __device__ __forceinline__ void viscous_face(const KP& p, int i, int j, int k, i
 #pragma unroll
             for (int q = 0; q < NV; ++q) w[q] = T(wd[q]);
         } else {
-            load_prim<NV, T>(p, cR, w);
+            load_prim<NV, T>(p, pidx_d(p, D, i, j, k, t), w);
         }
 #pragma unroll
         for (int a = 0; a < ND; ++a) vR[a] = w[mom(a)];
@@ -817,15 +851,22 @@ __global__ __launch_bounds__(256) void k_audit(KP p) {
     }
     const float z = mlp_logit_wave(p.mlp, x);
     const bool pred = z > p.mlp->thr;
-    // counts of the wavefront, then one atomic per wavefront and class
+    // counts of the wavefront (ballots), of the block (LDS), then one atomic per block and class into the block's
+    // slot row
+    __shared__ unsigned cnt[4];
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
+    __syncthreads();
     const unsigned long long bal_tp = __ballot(upgrade && label && pred), bal_fp = __ballot(upgrade && !label && pred),
                              bal_fn = __ballot(upgrade && label && !pred), bal_tn = __ballot(upgrade && !label && !pred);
     if ((threadIdx.x & 63) == 0) {
-        if (bal_tp) atomicAdd(&p.ind->last[0], (unsigned long long)__popcll(bal_tp));
-        if (bal_fp) atomicAdd(&p.ind->last[1], (unsigned long long)__popcll(bal_fp));
-        if (bal_fn) atomicAdd(&p.ind->last[2], (unsigned long long)__popcll(bal_fn));
-        if (bal_tn) atomicAdd(&p.ind->last[3], (unsigned long long)__popcll(bal_tn));
+        if (bal_tp) atomicAdd(&cnt[0], (unsigned)__popcll(bal_tp));
+        if (bal_fp) atomicAdd(&cnt[1], (unsigned)__popcll(bal_fp));
+        if (bal_fn) atomicAdd(&cnt[2], (unsigned)__popcll(bal_fn));
+        if (bal_tn) atomicAdd(&cnt[3], (unsigned)__popcll(bal_tn));
     }
+    __syncthreads();
+    if (threadIdx.x < 4 && cnt[threadIdx.x])
+        atomicAdd(&p.ind->part[blockIdx.x % kAuditSlots][threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
 }
 
 // Training samples of one axis: the features (16 floats per face, the 16th = 1) and the label value of every face
@@ -1151,13 +1192,19 @@ __global__ void k_indicator_control(KP p, int op) {
         if (p.st->done) return;
         d->pending = 1;
         if (audit_step)
-            for (int q = 0; q < 4; ++q) d->last[q] = 0;
+            for (int r = 0; r < kAuditSlots; ++r)
+                for (int q = 0; q < 4; ++q) d->part[r][q] = 0;
         return;
     }
     if (!d->pending) return;
     d->pending = 0;
     if (audit_step) {
-        for (int q = 0; q < 4; ++q) d->total[q] += d->last[q];
+        for (int q = 0; q < 4; ++q) {
+            unsigned long long s = 0;
+            for (int r = 0; r < kAuditSlots; ++r) s += d->part[r][q];
+            d->last[q] = s;
+            d->total[q] += s;
+        }
         d->audits += 1;
         const double troubled = double(d->last[0] + d->last[2]), missed = double(d->last[2]);
         const double faces = double(d->last[0] + d->last[1] + d->last[2] + d->last[3]);

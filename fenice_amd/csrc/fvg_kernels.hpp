@@ -42,6 +42,13 @@ struct KP {
     int viscous;
     double mu, k_cond, inv_h2;
     int vper[3];    // periodic axes of the viscous stencil
+    // padded primitive planes: W carries PH halo layers on each side of every axis (none along z in 2-D), filled
+    // by k_cells with the value a reconstruction stencil takes there (periodic wrap, ghost slab or edge copy), so
+    // stencils read W without boundary logic. Cell (i, j, k), coordinates from -PH, is element
+    // ((k + phz) * py + j + PH) * px + i + PX0 of a plane of npad elements; rows are PXA elements apart in
+    // multiples and start their interior at PX0, so that every row's interior starts on a cache-line boundary.
+    int px, py, phz;
+    long long npad;
     // buffers (plane sets)
     // W, G, Fc and Fv hold doubles in the FP64 mode and floats in the FP32-flux mode (flux_precision)
     const double* U;    // stage input state (always FP64)
@@ -60,6 +67,10 @@ struct KP {
     double ind_fn_limit;      // missed fraction of troubled faces that switches the indicator off
     double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };
+
+constexpr int PH = 3;     // halo layers of the padded primitive planes (the WENO5-Z stencil reaches 3 cells out)
+constexpr int PXA = 32;   // row alignment of the padded planes in elements (128 B in FP32, 256 B in FP64)
+constexpr int PX0 = 32;   // element of a padded row that holds i = 0 (the low x halo sits just before it)
 
 enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3 };
 enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };

@@ -42,7 +42,11 @@ struct IndState {
     int off;                       // 1: the audit found too many missed troubled faces; WENO5-Z everywhere
     int audits;
     int pending;                   // the current step is being taken
+    // per-block audit counts land in one of kAuditSlots slot rows (block index mod kAuditSlots), so that the atomics
+    // of the blocks of one audit spread over many addresses; the control kernel sums the rows into last[]
+    unsigned long long part[64][4];
 };
+constexpr int kAuditSlots = 64;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -71,9 +75,10 @@ template <typename T> __device__ __forceinline__ T upwind5(T c0, T c1, T c2, T c
     return (T(2.0) * c0 - T(13.0) * c1 + T(47.0) * c2 + T(27.0) * c3 - T(3.0) * c4) * T(1.0 / 60.0);
 }
 
-// max_k |w_k - d_k| of the WENO5-Z weights of one face value
-template <typename T> __device__ __forceinline__ T weno5z_departure(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
-    const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2) + weno_floor<T>();
+// max_k |w_k - d_k| of the WENO5-Z weights of one face value, with the regularisation eps_rel * m + floor, where m
+// is the mean square of the five values (density, pressure) or the face's kinetic-plus-acoustic scale (velocities)
+template <typename T> __device__ __forceinline__ T weno5z_departure(T c0, T c1, T c2, T c3, T c4, T eps_rel, T m) {
+    const T eps = eps_rel * m + weno_floor<T>();
     const T s0 = c0 - T(2.0) * c1 + c2, t0 = c0 - T(4.0) * c1 + T(3.0) * c2;
     const T s1 = c1 - T(2.0) * c2 + c3, t1 = c1 - c3;
     const T s2 = c2 - T(2.0) * c3 + c4, t2 = T(3.0) * c2 - T(4.0) * c3 + c4;
@@ -87,13 +92,31 @@ template <typename T> __device__ __forceinline__ T weno5z_departure(T c0, T c1, 
     return smax(smax(fabs(a0 * inv - T(0.1)), fabs(a1 * inv - T(0.6))), fabs(a2 * inv - T(0.3)));
 }
 
-// label value of a face: the largest departure over the variables and both face states
+template <typename T> __device__ __forceinline__ T mean_square5(T c0, T c1, T c2, T c3, T c4) {
+    return (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2);
+}
+
+// label value of a face: the largest departure over the variables and both face states. A velocity component is
+// regularised by the face's scale mean(|v|^2 + p / rho) over its six cells rather than by its own mean square, so
+// that a component near zero (where rounding of the much larger pressure flux leaves noise of ~1e-7) does not count
+// as a discontinuity; density and pressure keep their own mean square, as WENO5-Z itself regularises them.
 template <int NV, typename T> __device__ __forceinline__ T face_departure(const T (*c)[NV], T eps_rel) {
+    T vs = T(0);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        T v2 = c[s][1] * c[s][1] + c[s][2] * c[s][2];
+        if constexpr (NV == 5) v2 += c[s][4] * c[s][4];
+        vs += v2 + dv(c[s][3], c[s][0]);
+    }
+    vs = vs * T(1.0 / 6.0);
     T d = T(0);
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        d = smax(d, weno5z_departure(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], eps_rel));
-        d = smax(d, weno5z_departure(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], eps_rel));
+        const bool vel = q == 1 || q == 2 || q == 4;
+        const T ml = vel ? vs : mean_square5(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q]);
+        const T mr = vel ? vs : mean_square5(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q]);
+        d = smax(d, weno5z_departure(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], eps_rel, ml));
+        d = smax(d, weno5z_departure(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], eps_rel, mr));
     }
     return d;
 }

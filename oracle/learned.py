@@ -6,7 +6,10 @@ The product (fenice_amd) never imports it.
 The indicator has no reference counterpart beyond the functions its labels come from:
 - label: the departure of the WENO5-Z weights from the linear weights (0.1, 0.6, 0.3) (weno5z, Schemes.cpp:338-353),
   max over the primitive variables and both face states of a face, max_k |w_k - d_k| with
-  w_k = a_k / (a0 + a1 + a2); a face is troubled when it exceeds the label threshold;
+  w_k = a_k / (a0 + a1 + a2); a face is troubled when it exceeds the label threshold. Density and pressure are
+  regularised as WENO5-Z regularises them (eps_rel times their mean square); velocity components by eps_rel times
+  the face's mean |v|^2 + p / rho over its six cells, so that rounding noise in a component that is locally near
+  zero does not count as a jump;
 - smooth faces take the linear upwind-5 value (2c0 - 13c1 + 47c2 + 27c3 - 3c4) / 60, which is weno5z with a = d
   (exact for quartic cell-average data);
 - features of a face (15): for density, pressure and the velocity component with the largest total variation over
@@ -61,8 +64,11 @@ def stencils(W, axis):
     return np.stack([c.reshape(W.shape[0], -1) for c in cells])
 
 
-def weno5z_weights(c0, c1, c2, c3, c4, eps_rel):
-    eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * 0.2 + 1e-300
+def weno5z_weights(c0, c1, c2, c3, c4, eps_rel, m=None):
+    """WENO5-Z weights with the regularisation eps_rel * m (m: the mean square of the five values by default)."""
+    if m is None:
+        m = (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * 0.2
+    eps = eps_rel * m + 1e-300
     s0, t0 = c0 - 2.0 * c1 + c2, c0 - 4.0 * c1 + 3.0 * c2
     s1, t1 = c1 - 2.0 * c2 + c3, c1 - c3
     s2, t2 = c2 - 2.0 * c3 + c4, 3.0 * c2 - 4.0 * c3 + c4
@@ -74,13 +80,28 @@ def weno5z_weights(c0, c1, c2, c3, c4, eps_rel):
     return a / a.sum(axis=0)
 
 
+def velocity_scale(st, nvel):
+    """mean over the six cells of |v|^2 + p / rho: the regularisation scale of the velocity components. st: (6, nv, nf)."""
+    vel = [1, 2] if nvel == 2 else [1, 2, 4]
+    v2 = sum(st[:, q] ** 2 for q in vel)
+    return (v2 + st[:, 3] / st[:, 0]).mean(axis=0)
+
+
 def departure(st, eps_rel, nvel):
-    """Label value of each face: max over variables and both face states of max_k |w_k - d_k|. st: (6, nv, nf)."""
+    """Label value of each face: max over variables and both face states of max_k |w_k - d_k|. st: (6, nv, nf).
+    Density and pressure are regularised by their own mean square (as WENO5-Z is), velocity components by the
+    face's kinetic-plus-acoustic scale (velocity_scale), so rounding noise in a near-zero component is no jump."""
     c = st
-    wl = weno5z_weights(c[0], c[1], c[2], c[3], c[4], eps_rel)
-    wr = weno5z_weights(c[5], c[4], c[3], c[2], c[1], eps_rel)
-    d = np.maximum(np.abs(wl - D_LIN[:, None, None]).max(axis=0), np.abs(wr - D_LIN[:, None, None]).max(axis=0))
-    return d.max(axis=0)
+    vs = velocity_scale(st, nvel)
+    vel = [1, 2] if nvel == 2 else [1, 2, 4]
+    out = np.zeros(st.shape[2])
+    for q in range(st.shape[1]):
+        m = vs if q in vel else None
+        wl = weno5z_weights(c[0, q], c[1, q], c[2, q], c[3, q], c[4, q], eps_rel, m)
+        wr = weno5z_weights(c[5, q], c[4, q], c[3, q], c[2, q], c[1, q], eps_rel, m)
+        d = np.maximum(np.abs(wl - D_LIN[:, None]).max(axis=0), np.abs(wr - D_LIN[:, None]).max(axis=0))
+        out = np.maximum(out, d)
+    return out
 
 
 def upwind5(c0, c1, c2, c3, c4):

@@ -42,6 +42,21 @@ def test_departure_closed_forms():
     assert L.departure(flat, 1e-6, 3).max() == 0.0
 
 
+def test_departure_ignores_rounding_noise_in_a_near_zero_velocity():
+    # a velocity component that is zero up to noise of 1e-7 beside a pressure of 71 (Mach 0.1): WENO5-Z regularised by
+    # the component's own mean square sees jumps; the label, regularised by |v|^2 + p / rho, does not; a real jump of
+    # that component (0.3, a tenth of the sound speed) still counts
+    rng = np.random.default_rng(3)
+    st = np.zeros((6, 5, 200))
+    st[:, 0], st[:, 3] = 1.0, 71.0
+    st[:, 4] = 1e-7 * rng.standard_normal((6, 200))
+    own = np.array([L.weno5z_weights(*[st[s, 4] for s in range(5)], 1e-6) for _ in range(1)])[0]
+    assert np.abs(own - L.D_LIN[:, None]).max() > 0.25
+    assert L.departure(st, 1e-6, 3).max() < 1e-3
+    st[3:, 4] += 0.3
+    assert L.departure(st, 1e-6, 3).min() > 0.25
+
+
 def test_feature_invariances():
     rng = np.random.default_rng(1)
     st = 1.0 + 0.1 * rng.random((6, 5, 50))
