@@ -65,6 +65,31 @@ __device__ __forceinline__ long long xcd_block() {
     return (long long)(b & 7u) * per + (b >> 3);
 }
 
+__device__ __forceinline__ unsigned mdiv(unsigned n, Magic g) { return (__umulhi(n, g.m) + n) >> g.l; }
+
+// The row and chunk of a row block (a grid of nplanes * ny * nchunk blocks of kRowBlock threads; the row of
+// (j, k) holds nx elements). Blocks are dispatched round-robin over the 8 XCDs; when ny divides by 8, XCD x takes
+// the rows j in [x ny/8, (x+1) ny/8) of every plane, plane after plane, so the few planes a z-stencil spans of its
+// slab stay in that XCD's L2. Otherwise the rows go in mesh order.
+struct RowBlock {
+    int i, j, k;   // this thread's x position (may be >= nx in the last chunk), row and plane
+};
+__device__ __forceinline__ RowBlock row_block(const KP& p) {
+    const unsigned b = blockIdx.x;
+    unsigned m = b, j0 = 0;
+    Magic rows = p.mg_ny;
+    unsigned nrow = p.ny;
+    if (p.slabs) {
+        m = b >> 3;
+        nrow = p.ny >> 3;
+        j0 = (b & 7u) * nrow;
+        rows = p.mg_slab;
+    }
+    const unsigned r = mdiv(m, p.mg_chunk), chunk = m - r * p.nchunk;
+    const unsigned k = mdiv(r, rows), jj = r - k * nrow;
+    return {int(chunk * kRowBlock + threadIdx.x), int(j0 + jj), int(k)};
+}
+
 // Wave-level then block-level maximum of non-negative, non-NaN doubles, then one atomicMax of the bit pattern.
 __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
     __shared__ double red[16];
@@ -326,12 +351,11 @@ __device__ __forceinline__ void push_halo(const KP& p, int i, int j, int k, cons
 }
 template <int NV, typename T = double>
 __global__ __launch_bounds__(256) void k_cells(KP p) {
-    const long long c = xcd_block() * blockDim.x + threadIdx.x;
+    const RowBlock rb = row_block(p);
+    const int i = rb.i, j = rb.j, k = rb.k;
+    const long long c = ((long long)k * p.ny + j) * p.nx + i;
     double local = 0.0;
-    if (c < p.nc && !p.st->done) {
-        const int i = (int)(c % p.nx);
-        const long long r = c / p.nx;
-        const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+    if (i < p.nx && !p.st->done) {
         const long long pc = pidx(p, i, j, k);
         if constexpr (std::is_same<T, float>::value) {
             // FP32 fluxes: the primitives are the FP64 ones rounded to float (the viscous ghosts of a periodic side
@@ -716,12 +740,14 @@ template <int NV, int D, int MODE, typename T = double>
 __global__ __launch_bounds__(256) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
-    const long long f = xcd_block() * blockDim.x + threadIdx.x;
+    // z faces: row blocks (rows (j, k) of nx faces, k up to nz); x and y faces: one face per thread in face order
+    const RowBlock rb = D == 2 ? row_block(p) : RowBlock{0, 0, 0};
+    const long long f = D == 2 ? ((long long)rb.k * p.ny + rb.j) * p.nx + rb.i : xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
-    if (f < nf && !p.st->done) {
-        const int i = (int)(f % ni);
+    if ((D == 2 ? rb.i < p.nx : f < nf) && !p.st->done) {
+        const int i = D == 2 ? rb.i : (int)(f % ni);
         const long long r = f / ni;
-        const int j = (int)(r % nj), k = (int)(r / nj);
+        const int j = D == 2 ? rb.j : (int)(r % nj), k = D == 2 ? rb.k : (int)(r / nj);
         const int t = coord(D, i, j, k);
         const int n = ncell(p, D);
         T UL[NV], UR[NV];
@@ -812,10 +838,18 @@ __global__ __launch_bounds__(256) void k_faces_learned(KP p) {
     if (p.st->done) return;   // uniform: the whole grid leaves
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
-    const long long f = xcd_block() * blockDim.x + threadIdx.x;
-    const bool active = f < nf;
+    // z faces: row blocks (as k_faces)
+    const RowBlock rb = D == 2 ? row_block(p) : RowBlock{0, 0, 0};
+    const long long f = D == 2 ? ((long long)rb.k * p.ny + rb.j) * p.nx + rb.i : xcd_block() * blockDim.x + threadIdx.x;
+    const bool active = D == 2 ? rb.i < p.nx : f < nf;
     int i = 0, j = 0, k = 0, t = 0;
-    if (active) face_coords<NV, D>(p, f, i, j, k, t);
+    if (D == 2) {
+        i = rb.i;
+        j = rb.j;
+        k = t = rb.k;
+    } else if (active) {
+        face_coords<NV, D>(p, f, i, j, k, t);
+    }
     const int n = ncell(p, D);
     float F[NV], sm = 0.0f;
     learned_face<NV, D>(p, active, i, j, k, t, n, F, sm);
@@ -920,11 +954,10 @@ __device__ __forceinline__ void combine(const KP& p, long long c, const double* 
 
 template <int NV, int UPD, typename T = double>
 __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
-    const long long c = xcd_block() * blockDim.x + threadIdx.x;
-    if (c >= p.nc || p.st->done) return;
-    const int i = (int)(c % p.nx);
-    const long long r = c / p.nx;
-    const int j = (int)(r % p.ny), k = (int)(r / p.ny);
+    const RowBlock rb = row_block(p);
+    const int i = rb.i, j = rb.j, k = rb.k;
+    if (i >= p.nx || p.st->done) return;
+    const long long c = ((long long)k * p.ny + j) * p.nx + i;
     const long long fx = ((long long)k * p.ny + j) * (p.nx + 1) + i;
     const long long fy = ((long long)k * (p.ny + 1) + j) * p.nx + i;
     const long long fz = c;

@@ -24,6 +24,13 @@ struct DevState {
     int done, diverged, steps, max_steps;
 };
 
+// n / d for 0 <= n < 2^31 as (umulhi(n, m) + n) >> l, with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1
+// (the round-up method of Granlund and Montgomery, 1994); the sum stays below 2^32 for n < 2^31
+struct Magic {
+    unsigned m;
+    int l;
+};
+
 struct KP {
     // mesh
     int nx, ny, nz, is3d;
@@ -49,6 +56,10 @@ struct KP {
     // multiples and start their interior at PX0, so that every row's interior starts on a cache-line boundary.
     int px, py, phz;
     long long npad;
+    // row blocks (k_cells, k_update, z faces): blocks of kBlock threads over chunks of a row of nx elements;
+    // divisors by multiply-high (Magic): chunks per row, rows per XCD slab (ny / 8), rows per plane (ny)
+    int nchunk, slabs;   // slabs: 1 when ny divides by 8 (each XCD takes a y-slab), else 0 (rows in mesh order)
+    Magic mg_chunk, mg_slab, mg_ny;
     // buffers (plane sets)
     // W, G, Fc and Fv hold doubles in the FP64 mode and floats in the FP32-flux mode (flux_precision)
     const double* U;    // stage input state (always FP64)
@@ -68,6 +79,7 @@ struct KP {
     double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };
 
+constexpr int kRowBlock = 256;   // threads of a row block
 constexpr int PH = 3;     // halo layers of the padded primitive planes (the WENO5-Z stencil reaches 3 cells out)
 constexpr int PXA = 32;   // row alignment of the padded planes in elements (128 B in FP32, 256 B in FP64)
 constexpr int PX0 = 32;   // element of a padded row that holds i = 0 (the low x halo sits just before it)
