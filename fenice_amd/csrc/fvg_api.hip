@@ -583,6 +583,10 @@ void fill_kernel_params(const fvg_config& cfg, KP& p) {
     p.mg_chunk = magic_for((unsigned)p.nchunk);
     p.mg_ny = magic_for((unsigned)cfg.ny);
     p.mg_slab = magic_for(p.slabs ? (unsigned)(cfg.ny / 8) : 1u);
+    for (int d = 0; d < 3; ++d) {
+        p.mg_fi[d] = magic_for((unsigned)(cfg.nx + (d == 0)));
+        p.mg_fj[d] = magic_for((unsigned)(cfg.ny + (d == 1)));
+    }
     p.npad = (long long)p.px * p.py * (cfg.nz + 2 * p.phz);
     p.cfl = cfg.cfl;
     p.Ldt = fvg_dt_length(&cfg);
@@ -615,8 +619,11 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
     if (cfg.flux_precision == FVG_FLUX_FP32 && cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE)
         return set_err(nullptr, FVG_EUNSUPPORTED, "flux_precision = fp32 with MUSCL-Hancock is not on the device path");
-    const double faces = (double)(cfg.nx + 1) * cfg.ny * cfg.nz;
-    if (faces * 64 > 9.0e18) return set_err(nullptr, FVG_EINVAL, "mesh too large");
+    // face and row-block indices are 32-bit: every axis' face count stays below 2^31
+    const long long fmax = std::max({(long long)(cfg.nx + 1) * cfg.ny * cfg.nz, (long long)cfg.nx * (cfg.ny + 1) * cfg.nz,
+                                     (long long)cfg.nx * cfg.ny * (cfg.nz + 1)});
+    if (fmax >= (1LL << 31))
+        return set_err(nullptr, FVG_EUNSUPPORTED, "mesh too large: the faces of an axis must number below 2^31");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(nullptr, FVG_EHIP, "no HIP device");

@@ -736,6 +736,17 @@ __device__ __forceinline__ void face_states(const KP& p, int i, int j, int k, in
     }
 }
 
+// (i, j, k) and the coordinate t along D of face f (< 2^31) normal to D, in that axis' face order
+template <int NV, int D>
+__device__ __forceinline__ void face_coords(const KP& p, long long f, int& i, int& j, int& k, int& t) {
+    const unsigned fu = (unsigned)f, ni = p.nx + (D == 0), nj = p.ny + (D == 1);
+    const unsigned r = mdiv(fu, p.mg_fi[D]), kk = mdiv(r, p.mg_fj[D]);
+    i = (int)(fu - r * ni);
+    j = (int)(r - kk * nj);
+    k = (int)kk;
+    t = coord(D, i, j, k);
+}
+
 template <int NV, int D, int MODE, typename T = double>
 __global__ __launch_bounds__(256) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
@@ -745,10 +756,8 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
     const long long f = D == 2 ? ((long long)rb.k * p.ny + rb.j) * p.nx + rb.i : xcd_block() * blockDim.x + threadIdx.x;
     double local = 0.0;
     if ((D == 2 ? rb.i < p.nx : f < nf) && !p.st->done) {
-        const int i = D == 2 ? rb.i : (int)(f % ni);
-        const long long r = f / ni;
-        const int j = D == 2 ? rb.j : (int)(r % nj), k = D == 2 ? rb.k : (int)(r / nj);
-        const int t = coord(D, i, j, k);
+        int i = rb.i, j = rb.j, k = rb.k, t = rb.k;
+        if (D != 2) face_coords<NV, D>(p, f, i, j, k, t);
         const int n = ncell(p, D);
         T UL[NV], UR[NV];
         face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
@@ -765,16 +774,6 @@ __global__ __launch_bounds__(256) void k_faces(KP p) {
 // the linear upwind-5 value; the others, and every face once the indicator is off, take WENO5-Z. Faces without a
 // full stencil keep their first-order states as in k_faces.
 // ------------------------------------------------------------------------------------------------------------
-template <int NV, int D>
-__device__ __forceinline__ void face_coords(const KP& p, long long f, int& i, int& j, int& k, int& t) {
-    const int ni = p.nx + (D == 0), nj = p.ny + (D == 1);
-    i = (int)(f % ni);
-    const long long r = f / ni;
-    j = (int)(r % nj);
-    k = (int)(r / nj);
-    t = coord(D, i, j, k);
-}
-
 template <int NV, int D>
 __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
     const int n = ncell(p, D);

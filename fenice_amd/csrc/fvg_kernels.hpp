@@ -60,6 +60,7 @@ struct KP {
     // divisors by multiply-high (Magic): chunks per row, rows per XCD slab (ny / 8), rows per plane (ny)
     int nchunk, slabs;   // slabs: 1 when ny divides by 8 (each XCD takes a y-slab), else 0 (rows in mesh order)
     Magic mg_chunk, mg_slab, mg_ny;
+    Magic mg_fi[3], mg_fj[3];   // faces per row and rows per plane of the faces normal to each axis
     // buffers (plane sets)
     // W, G, Fc and Fv hold doubles in the FP64 mode and floats in the FP32-flux mode (flux_precision)
     const double* U;    // stage input state (always FP64)
