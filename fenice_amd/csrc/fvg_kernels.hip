@@ -748,7 +748,7 @@ __device__ __forceinline__ void face_coords(const KP& p, long long f, int& i, in
 }
 
 template <int NV, int D, int MODE, typename T = double>
-__global__ __launch_bounds__(256) void k_faces(KP p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_faces(KP p) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
     // z faces: row blocks (rows (j, k) of nx faces, k up to nz); x and y faces: one face per thread in face order
@@ -833,7 +833,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, int i, in
 }
 
 template <int NV, int D>
-__global__ __launch_bounds__(256) void k_faces_learned(KP p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_faces_learned(KP p) {
     if (p.st->done) return;   // uniform: the whole grid leaves
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
