@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, published (FMA counted as 2)
 BYTES_PER_CELL_STAGE_RK3_3D = 8.0 * 5 * (2 + 3 + 3) / 3.0   # 106.7 B: stage input, U^n on stages 2-3, output
+# HBM bytes of one stage of the 256^3 workload (k_cells + k_faces x/y/z + k_update, the update averaged over the
+# three SSP-RK3 stages), from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB
+# device copy reads in the same runs); the runs and per-kernel figures are in profiles/tgv256_weno5_rk3_<flux>.md
+MEASURED_STAGE_TRAFFIC = {("fp64", 256): 16.594e9, ("fp32", 256): 7.662e9}
 
 from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
 from fenice_amd.workloads import taylor_green_keys as run_keys  # noqa: E402
@@ -285,7 +289,8 @@ def main():
         "l1_error_vs_fp64_oracle": 0.0 if a.flux == "fp64" else None,
         "roofline": {"bound": "hbm", "kernel": "one RK stage (k_cells, k_faces x/y/z, k_control, k_update)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "bytes_per_cell_stage": round(BYTES_PER_CELL_STAGE_RK3_3D, 2)},
+                     "traffic": MEASURED_STAGE_TRAFFIC.get((a.flux, n)) if a.indicator == "none" else None,
+                     "bytes_per_cell_stage": round(BYTES_PER_CELL_STAGE_RK3_3D, 2)},
         "kernels": {k: {"ms_total": round(ms, 3), "launches": cnt, "avg_ms": round(ms / cnt, 4)} for k, (ms, cnt) in fam.items()},
         "dominant_kernel": dom[0],
         "conservation": {"mass_rel_drift": abs(tot1[0] - tot0[0]) / abs(tot0[0]), "energy_rel_drift": abs(tot1[2] - tot0[2]) / abs(tot0[2])},

@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libfvg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FVG_ARCH", "gfx950")
 
-SOURCES = ["fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip", "fvg_config.cpp"]
+SOURCES = ["fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip", "fvg_config.cpp", "fvg_checkpoint.cpp",
+           "fvg_npz.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-I" + os.path.join(ROOT, "include")]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -20,7 +20,7 @@ FLUX_FP64, FLUX_FP32 = 0, 1
 INDICATOR_NONE, INDICATOR_LEARNED = 0, 1
 
 # status codes
-OK, EINVAL, ELOGIC, EDIVERGED, EHIP, ENOMEM, EUNSUPPORTED = range(7)
+OK, EINVAL, ELOGIC, EDIVERGED, EHIP, ENOMEM, EUNSUPPORTED, EIO = range(8)
 
 
 class Config(C.Structure):

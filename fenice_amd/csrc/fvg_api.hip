@@ -866,6 +866,32 @@ int fvg_set_lagged_speed(fvg_ctx* c, double sp) {
     return FVG_OK;
 }
 
+int fvg_checkpoint_write(fvg_ctx* c, const char* path, double t, int step) {
+    if (!c || !path) return set_err(c, FVG_EINVAL, "null argument");
+    const int nq = c->NV;   // the reference's columns: 4 in 2-D, 5 in 3-D
+    std::vector<double> U((size_t)nq * c->nc);
+    int rc = fvg_download(c, U.data(), (size_t)c->nc, nq);
+    if (rc) return rc;
+    const double s = c->cfg.time_scheme == FVG_TS_HANCOCK ? c->st_host->lagged : 0.0;
+    char err[512] = {0};
+    rc = fvg_checkpoint_save(&c->cfg, U.data(), (size_t)c->nc, nq, t, step, s, path, err, sizeof err);
+    return rc ? set_err(c, rc, err) : FVG_OK;
+}
+
+int fvg_checkpoint_read(fvg_ctx* c, const char* path, double* t, int* step) {
+    if (!c || !path || !t || !step) return set_err(c, FVG_EINVAL, "null argument");
+    const int nq = c->NV;
+    std::vector<double> U((size_t)nq * c->nc);
+    double s = 0.0;
+    char err[512] = {0};
+    int rc = fvg_checkpoint_load(&c->cfg, path, U.data(), (size_t)c->nc, nq, t, step, &s, err, sizeof err);
+    if (rc) return set_err(c, rc, err);
+    if ((rc = fvg_upload(c, U.data(), (size_t)c->nc, nq))) return rc;
+    // a file without S leaves the lagged speed unset: the next MUSCL-Hancock step seeds it (SimulationRestart.cpp:343-344)
+    if (c->cfg.time_scheme == FVG_TS_HANCOCK) return fvg_set_lagged_speed(c, s);
+    return FVG_OK;
+}
+
 int fvg_conserved_totals(fvg_ctx* c, double out[3]) {
     if (!c || !out) return set_err(c, FVG_EINVAL, "null argument");
     HIPCHK(c, hipSetDevice(c->device));

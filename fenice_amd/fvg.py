@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .config import Config, EDIVERGED, EHIP, EINVAL, ELOGIC, ENOMEM, EUNSUPPORTED, OK
+from .config import Config, EDIVERGED, EHIP, EINVAL, EIO, ELOGIC, ENOMEM, EUNSUPPORTED, OK
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvg.so")
@@ -31,6 +31,7 @@ EXPORTS = [
     "fvg_batch_run", "fvg_batch_set_block", "fvg_batch_get_block", "fvg_batch_tune",
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
     "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
+    "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read",
 ]
 
 MLP_NPARAM = 1601
@@ -65,8 +66,12 @@ class InvalidArgument(FvgError, ValueError):
     code = EINVAL
 
 
+class CheckpointIOError(FvgError, OSError):
+    code = EIO
+
+
 _ERR = {EINVAL: InvalidArgument, ELOGIC: LogicError, EDIVERGED: Diverged, EHIP: HipError, ENOMEM: OutOfMemory,
-        EUNSUPPORTED: Unsupported}
+        EUNSUPPORTED: Unsupported, EIO: CheckpointIOError}
 
 _lib = None
 
@@ -123,6 +128,11 @@ def lib():
         L.fvg_trainer_grad.argtypes = [C.c_void_p, F, C.c_float, F, _D]
         L.fvg_trainer_fit.argtypes = [C.c_void_p, F, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, _D]
         L.fvg_mlp_logits.argtypes = [C.c_int, F, C.c_float, F, C.c_longlong, F]
+        L.fvg_checkpoint_save.argtypes = [P, _D, C.c_size_t, C.c_int, C.c_double, C.c_int, C.c_double, C.c_char_p, C.c_char_p,
+                                          C.c_size_t]
+        L.fvg_checkpoint_load.argtypes = [P, C.c_char_p, _D, C.c_size_t, C.c_int, _D, _I, _D, C.c_char_p, C.c_size_t]
+        L.fvg_checkpoint_write.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int]
+        L.fvg_checkpoint_read.argtypes = [C.c_void_p, C.c_char_p, _D, _I]
         _lib = L
     return _lib
 
@@ -237,6 +247,16 @@ class Solver:
     @lagged_speed.setter
     def lagged_speed(self, s: float):
         self._c(lib().fvg_set_lagged_speed(self._h, s))
+
+    def checkpoint_write(self, path: str, t: float, step: int):
+        """Write the state (and the MUSCL-Hancock lagged speed) as the reference's restart file (fvg_checkpoint_write)."""
+        self._c(lib().fvg_checkpoint_write(self._h, os.fsencode(path), t, step))
+
+    def checkpoint_read(self, path: str):
+        """Load a restart file (the reference's or checkpoint_write's) into the context; returns (t, step)."""
+        t, n = C.c_double(), C.c_int()
+        self._c(lib().fvg_checkpoint_read(self._h, os.fsencode(path), C.byref(t), C.byref(n)))
+        return t.value, n.value
 
     def conserved_totals(self):
         out = np.empty(3)
@@ -448,3 +468,29 @@ def mlp_logits(params, X, threshold: float = 0.0, device: int = 0):
     if rc != OK:
         raise _ERR.get(rc, FvgError)(lib().fvg_trainer_last_error(None).decode(errors="replace"))
     return z
+
+
+def _host_err(rc: int, err) -> None:
+    if rc != OK:
+        raise _ERR.get(rc, FvgError)(err.value.decode(errors="replace") or f"status {rc}")
+
+
+def checkpoint_save(cfg: Config, U, t: float, step: int, path: str, lagged_speed: float = 0.0) -> None:
+    """Write U (ncells x 4 in 2-D, x 5 in 3-D) with t, step and lagged_speed as a restart file (host only,
+    fvg_checkpoint_save)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    err = C.create_string_buffer(512)
+    _host_err(lib().fvg_checkpoint_save(C.byref(cfg), _dp(U), U.shape[0], U.shape[1], t, step, lagged_speed, os.fsencode(path),
+                                        err, 512), err)
+
+
+def checkpoint_load(cfg: Config, path: str, ncomp: int | None = None):
+    """Read a restart file of cfg's mesh (host only, fvg_checkpoint_load): (U, t, step, lagged_speed)."""
+    nc = cfg.nx * cfg.ny * cfg.nz
+    nq = ncomp or (5 if cfg.nz > 1 else 4)
+    U = np.empty((nc, nq))
+    t, n, s = C.c_double(), C.c_int(), C.c_double()
+    err = C.create_string_buffer(512)
+    _host_err(lib().fvg_checkpoint_load(C.byref(cfg), os.fsencode(path), _dp(U), nc, nq, C.byref(t), C.byref(n), C.byref(s), err, 512),
+              err)
+    return U, t.value, n.value, s.value

@@ -31,7 +31,8 @@ typedef enum fvg_status {
     FVG_EDIVERGED = 3,  /* non-finite wave speed or time step, or t + dt == t (check_progress, std::runtime_error) */
     FVG_EHIP = 4,       /* a HIP runtime call failed */
     FVG_ENOMEM = 5,     /* device memory could not be allocated */
-    FVG_EUNSUPPORTED = 6 /* a configuration the device path does not implement (the message names it) */
+    FVG_EUNSUPPORTED = 6, /* a configuration the device path does not implement (the message names it) */
+    FVG_EIO = 7          /* a file could not be read or written, or is not a checkpoint (std::runtime_error) */
 } fvg_status;
 
 /* Boundary-condition types, in the order of the reference's enum BCType (cpp/include/fv/Boundaries.hpp:15). */
@@ -144,6 +145,27 @@ int fvg_advance(fvg_ctx* ctx, int max_steps, double* t, double t_end, double dt_
 /* The MUSCL-Hancock lagged wave speed (TimeScheme.hpp:96-97; checkpoint key "S"). */
 int fvg_get_lagged_speed(const fvg_ctx* ctx, double* s);
 int fvg_set_lagged_speed(fvg_ctx* ctx, double s);
+
+/* ---- Checkpoints (the reference's restart files) -------------------------------------------------------------
+ * A checkpoint is the reference's uniform-mesh restart file: a .npz of stored .npy entries with U (ncells x 4 in
+ * 2-D, x 5 in 3-D; FP64, the AoS Field), t, step, format_version = 1, nx, ny, nz, Lx, Ly, Lz, the output index
+ * (collection_time, collection_file; empty here) and S, the MUSCL-Hancock lagged speed, when positive
+ * (Simulation::checkpoint_header / save_checkpoint / load_checkpoint, SimulationRestart.cpp:61-85, 162-190, 254-360).
+ * A run continued from a checkpoint takes the same steps, bit for bit, as the run that wrote it, on either side.
+ * Errors: FVG_EIO (file, format), FVG_EINVAL (a mesh or size that differs from cfg / the context's). Entries and
+ * offsets of 4 GiB and more carry zip64 fields; the environment variable FVG_NPZ_ZIP64_FROM (bytes) lowers that
+ * size, so that tests can exercise them on small files. */
+/* Writes U (ncells x ncomp, ncomp 4 or 5) with t, step and lagged_speed (written as S when > 0) as `path`,
+ * through a temporary file renamed over it. Host only; err (may be NULL) gets the message. */
+int fvg_checkpoint_save(const fvg_config* cfg, const double* U, size_t ncells, int ncomp, double t, int step, double lagged_speed,
+                        const char* path, char* err, size_t err_len);
+/* Reads a checkpoint of a cfg mesh into U (ncells x ncomp; a file with 4 columns gives rho w = 0 in 3-D), t, step
+ * and lagged_speed (0 when the file has no S: the next MUSCL-Hancock step seeds it). Host only. */
+int fvg_checkpoint_load(const fvg_config* cfg, const char* path, double* U, size_t ncells, int ncomp, double* t, int* step,
+                        double* lagged_speed, char* err, size_t err_len);
+/* The context's state and lagged speed to / from a checkpoint (fvg_download / fvg_upload around the two above). */
+int fvg_checkpoint_write(fvg_ctx* ctx, const char* path, double t, int step);
+int fvg_checkpoint_read(fvg_ctx* ctx, const char* path, double* t, int* step);
 
 /* Volume averages of mass, kinetic energy and total energy (Simulation::volume_averages,
  * SimulationRun.cpp:45-64). The device sum is a fixed-order FP64 tree, so it matches the reference's

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <filesystem>
 #include <memory>
 #include <string>
 
@@ -127,6 +128,41 @@ int ref_run(void* hp, double* t, double t_end, int max_steps) {
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1000000000;
+    }
+}
+
+// The run's restart file (Simulation::checkpoint_path) into buf (cap bytes); its length, or -1 when it does not fit.
+int ref_checkpoint_path(void* hp, char* buf, int cap) {
+    const std::string p = static_cast<Handle*>(hp)->sim.checkpoint_path();
+    if ((int)p.size() + 1 > cap) return -1;
+    std::memcpy(buf, p.c_str(), p.size() + 1);
+    return (int)p.size();
+}
+
+// Simulation::save_checkpoint of the current state at time t as step `step` (into checkpoint_path()).
+int ref_save_checkpoint(void* hp, double t, int step) {
+    try {
+        Handle* h = static_cast<Handle*>(hp);
+        std::filesystem::create_directories(std::filesystem::path(h->sim.checkpoint_path()).parent_path());
+        h->sim.t = t;
+        h->sim.save_checkpoint(step);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Simulation::load_checkpoint from checkpoint_path(): state, t and lagged speed; returns the step, -1 on an error.
+int ref_load_checkpoint(void* hp, double* t) {
+    try {
+        Handle* h = static_cast<Handle*>(hp);
+        const int step = h->sim.load_checkpoint();
+        *t = h->sim.t;
+        return step;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
     }
 }
 

@@ -118,6 +118,9 @@ class Reference:
         L.ref_get_lagged_speed.restype = C.c_double
         L.ref_set_lagged_speed.argtypes = [C.c_void_p, C.c_double]
         L.ref_face_flux.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _I, C.c_longlong, _D, _D]
+        L.ref_checkpoint_path.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.ref_save_checkpoint.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.ref_load_checkpoint.argtypes = [C.c_void_p, _D]
 
     def create(self, case: str, **keys) -> "RefRun":
         ks = [k.encode() for k in keys]
@@ -186,6 +189,26 @@ class RefRun:
         t = C.c_double(0.0)
         steps = self.lib.ref_run(self.h, C.byref(t), t_end, max_steps)
         return t.value, steps
+
+    @property
+    def checkpoint_path(self) -> str:
+        buf = C.create_string_buffer(4096)
+        if self.lib.ref_checkpoint_path(self.h, buf, 4096) < 0:
+            raise RuntimeError("checkpoint path too long")
+        return buf.value.decode()
+
+    def save_checkpoint(self, t, step):
+        """Simulation::save_checkpoint (the reference's own writer) into checkpoint_path."""
+        if self.lib.ref_save_checkpoint(self.h, t, step) != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+
+    def load_checkpoint(self):
+        """Simulation::load_checkpoint (the reference's own reader) from checkpoint_path: (t, step)."""
+        t = C.c_double()
+        step = self.lib.ref_load_checkpoint(self.h, C.byref(t))
+        if step < 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+        return t.value, step
 
     @property
     def lagged_speed(self):
