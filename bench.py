@@ -198,7 +198,7 @@ def side_workload(name: str) -> dict:
     """Throughput of the other configurations of the path (parity cases, not the headline): the 1-D ensemble
     (1024 Riemann problems of 128..8192 cells to t = 0.15, one launch after the autotuner), the isentropic vortex at
     256^2 under MUSCL-Hancock (CFL 0.7, 200 steps, graph replay), and the Kelvin-Helmholtz layer at 2048^2
-    (WENO5-Z + SSP-RK3, 30 steps)."""
+    (WENO5-Z + SSP-RK3, 30 steps), and the 512^3 Taylor-Green vortex in its three precision modes."""
     from fenice_amd import fvg
     from fenice_amd.workloads import ensemble_keys, kelvin_helmholtz, kh_keys, riemann_ensemble
     if name == "ensemble":
@@ -235,6 +235,42 @@ def side_workload(name: str) -> dict:
         return {"workload": "Kelvin-Helmholtz 2048^2, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic, 30 steps",
                 "value": round(n * n * 3 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                 "ms_per_step": round(el / k * 1e3, 4), "steps": k}
+    if name == "tgv512":
+        # the 512^3 Taylor-Green vortex with FP32 fluxes and the learned indicator beside the plain FP64 path: each mode
+        # is timed on 8 steps after 1 warm-up step, then restarted from the initial field for 10 steps at a fixed dt
+        # (below the CFL step), whose density is compared with the FP64 run's (L1 and Linf, relative)
+        n = 512
+        U0 = tgv_state(n)
+        cells = n ** 3
+        modes, rho64 = {}, None
+        for label, flux, ind in (("fp64", "fp64", "none"), ("fp32", "fp32", "none"), ("fp32+learned", "fp32", "learned")):
+            with make_solver(n, flux, ind) as s:
+                s.upload(U0)
+                s.advance(0.0, 1e9, max_steps=1)
+                t0 = time.perf_counter()
+                _, k, _ = s.advance(0.0, 1e9, max_steps=8)
+                el = time.perf_counter() - t0
+                s.upload(U0)
+                t = 0.0
+                for _ in range(10):
+                    t += s.step(t, 1e9, 4e-4).dt
+                rho = s.download()[:, 0].copy()
+                m = {"value": round(cells * 3 * k / el / 1e6, 1), "ms_per_step": round(el / k * 1e3, 3), "steps": k}
+                if ind == "learned":
+                    _, total, off, audits = s.indicator_stats()
+                    m["indicator"] = {"audits": audits, "switched_off": off, "tp_fp_fn_tn_all_audits": total}
+            if rho64 is None:
+                rho64 = rho
+            else:
+                d = np.abs(rho - rho64)
+                m["error_vs_fp64"] = {"l1_rel": float(d.sum() / np.abs(rho64).sum()), "linf_rel": float(d.max() / np.abs(rho64).max()),
+                                      "steps": 10, "dt": 4e-4}
+            modes[label] = m
+        base = modes["fp64"]["value"]
+        for m in modes.values():
+            m["vs_fp64"] = round(m["value"] / base, 3)
+        return {"workload": "3-D Taylor-Green vortex 512^3, Navier-Stokes Re 1600 Pr 0.71 Mach 0.1, WENO5-Z + HLLC + SSP-RK3, "
+                            "CFL 0.5, periodic", "unit": "Mcell-updates/s per RK stage", "modes": modes}
     raise ValueError(name)
 
 
@@ -253,7 +289,7 @@ def main():
     ap.add_argument("--indicator", default="none", choices=["none", "learned"], help="troubled-face indicator (learned: fp32)")
     ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
-    ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh"],
+    ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
     if a.cpu_worker:

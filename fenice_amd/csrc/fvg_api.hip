@@ -36,7 +36,7 @@ using namespace fvg;
 namespace {
 
 constexpr int kBlock = 256;
-static_assert(kBlock == kRowBlock, "row-block kernels are launched with kBlock threads");
+static_assert(kBlock >= kRowBlock, "block sizes of the kernels' launch bounds");
 constexpr int kGraphSteps = 4;   // steps per captured graph of fvg_advance
 constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along x, 16 along z
 constexpr int kTotalsBlocks = 1024;
@@ -108,6 +108,8 @@ int padded_row(int nx) { return (PX0 + nx + PH + PXA - 1) / PXA * PXA; }
 unsigned row_grid(const fvg_ctx* c, int nplanes) { return (unsigned)((long long)nplanes * c->base.ny * c->base.nchunk); }
 unsigned cells_grid(const fvg_ctx* c) { return row_grid(c, c->base.nz); }
 unsigned faces_grid(const fvg_ctx* c, int d) { return d == 2 ? row_grid(c, c->base.nz + 1) : grid_for(c->nfa[d]); }
+unsigned cells_block(const fvg_ctx* c) { return (unsigned)c->base.rbt; }
+unsigned faces_block(const fvg_ctx* c, int d) { return d == 2 ? (unsigned)c->base.rbt : (unsigned)kBlock; }
 
 hipEvent_t take_event(fvg_ctx* c) {
     hipEvent_t e;
@@ -163,22 +165,22 @@ template <int NV, typename T>
 int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
     const KP p = make_kp(c, Uin);
     int rc;
-    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(kBlock), 0, c->stream, p); })))
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     if constexpr (std::is_same<T, float>::value) {
         if (mode == MODE_LEARNED) {
-#define LFACES(KERNEL, FAM, D, GRID) \
-    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(GRID), dim3(kBlock), 0, c->stream, p); })
+#define LFACES(KERNEL, FAM, D, GRID, BLOCK) \
+    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(GRID), dim3(BLOCK), 0, c->stream, p); })
             if (audit) {
-                if ((rc = LFACES(k_audit, FAM_AUDIT, 0, grid_for(c->nfa[0]))) || (rc = LFACES(k_audit, FAM_AUDIT, 1, grid_for(c->nfa[1]))))
+                if ((rc = LFACES(k_audit, FAM_AUDIT, 0, grid_for(c->nfa[0]), kBlock)) || (rc = LFACES(k_audit, FAM_AUDIT, 1, grid_for(c->nfa[1]), kBlock)))
                     return rc;
                 if constexpr (NV == 5)
-                    if ((rc = LFACES(k_audit, FAM_AUDIT, 2, grid_for(c->nfa[2])))) return rc;
+                    if ((rc = LFACES(k_audit, FAM_AUDIT, 2, grid_for(c->nfa[2]), kBlock))) return rc;
             }
-            if ((rc = LFACES(k_faces_learned, FAM_FACES_X, 0, faces_grid(c, 0))) || (rc = LFACES(k_faces_learned, FAM_FACES_Y, 1, faces_grid(c, 1))))
+            if ((rc = LFACES(k_faces_learned, FAM_FACES_X, 0, faces_grid(c, 0), faces_block(c, 0))) || (rc = LFACES(k_faces_learned, FAM_FACES_Y, 1, faces_grid(c, 1), faces_block(c, 1))))
                 return rc;
             if constexpr (NV == 5)
-                if ((rc = LFACES(k_faces_learned, FAM_FACES_Z, 2, faces_grid(c, 2)))) return rc;
+                if ((rc = LFACES(k_faces_learned, FAM_FACES_Z, 2, faces_grid(c, 2), faces_block(c, 2)))) return rc;
 #undef LFACES
             return FVG_OK;
         }
@@ -187,7 +189,7 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
         if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
             return rc;
 #define FACES(D, M)                                                                                                                       \
-    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, M, T>), dim3(faces_grid(c, D)), dim3(kBlock), 0, c->stream, p); })
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, M, T>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p); })
 #define AXIS(D) (mode == MODE_PLAIN ? FACES(D, MODE_PLAIN) : FACES(D, MODE_FIRST_ORDER))
     if ((rc = AXIS(0))) return rc;
     if ((rc = AXIS(1))) return rc;
@@ -203,7 +205,7 @@ template <int NV>
 int enqueue_hancock_faces(fvg_ctx* c, const KP& p) {
     int rc;
 #define HFACES(D)                                                                                                                         \
-    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, MODE_HANCOCK, double>), dim3(faces_grid(c, D)), dim3(kBlock), 0, \
+    launch(c, FAM_FACES_X + D, [&] { hipLaunchKernelGGL((k_faces<NV, D, MODE_HANCOCK, double>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, \
                                                         c->stream, p); })
     if ((rc = HFACES(0))) return rc;
     if ((rc = HFACES(1))) return rc;
@@ -217,7 +219,7 @@ template <int NV>
 int enqueue_hancock_residual(fvg_ctx* c, const double* Uin) {
     const KP p = make_kp(c, Uin);
     int rc;
-    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, double>), dim3(cells_grid(c)), dim3(kBlock), 0, c->stream, p); })))
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, double>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
         return rc;
@@ -245,7 +247,7 @@ int enqueue_control(fvg_ctx* c, int op, int arg) {
 template <int NV, typename T>
 int enqueue_update_nv(fvg_ctx* c, int upd, const double* U0, const double* Ut, double* out) {
     const KP p = c->base;
-    const dim3 g(cells_grid(c)), b(kBlock);
+    const dim3 g(cells_grid(c)), b(cells_block(c));
     return launch(c, FAM_UPDATE, [&] {
         switch (upd) {
         case UPD_RESIDUAL: hipLaunchKernelGGL((k_update<NV, UPD_RESIDUAL, T>), g, b, 0, c->stream, p, U0, Ut, out); break;
@@ -267,7 +269,7 @@ template <int NV, typename T>
 int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
     const KP p = make_kp(c, Uin);
     int rc;
-    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(kBlock), 0, c->stream, p); })))
+    if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     for (int d = 0; d < (NV == 5 ? 2 : 1); ++d) {
         rc = launch(c, FAM_FACES_X + d, [&] {
@@ -578,7 +580,8 @@ void fill_kernel_params(const fvg_config& cfg, KP& p) {
     p.px = padded_row(cfg.nx);
     p.py = cfg.ny + 2 * PH;
     p.phz = three ? PH : 0;
-    p.nchunk = (cfg.nx + kRowBlock - 1) / kRowBlock;
+    p.rbt = std::min(kRowBlock, (cfg.nx + 63) / 64 * 64);
+    p.nchunk = (cfg.nx + p.rbt - 1) / p.rbt;
     p.slabs = cfg.ny % 8 == 0;
     p.mg_chunk = magic_for((unsigned)p.nchunk);
     p.mg_ny = magic_for((unsigned)cfg.ny);
@@ -951,14 +954,14 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
         return set_err(c, FVG_ENOMEM, "hipMalloc of the samples");
     }
     const KP p = make_kp(c, c->U);
-    const dim3 gc(cells_grid(c)), gf((unsigned)((nfaces + kBlock - 1) / kBlock)), b(kBlock);
+    const dim3 gc(cells_grid(c)), bc(cells_block(c)), gf((unsigned)((nfaces + kBlock - 1) / kBlock)), b(kBlock);
     if (c->NV == 5) {
-        hipLaunchKernelGGL((k_cells<5, float>), gc, b, 0, c->stream, p);
+        hipLaunchKernelGGL((k_cells<5, float>), gc, bc, 0, c->stream, p);
         if (axis == 0) hipLaunchKernelGGL((k_samples<5, 0>), gf, b, 0, c->stream, p, dX, dd);
         else if (axis == 1) hipLaunchKernelGGL((k_samples<5, 1>), gf, b, 0, c->stream, p, dX, dd);
         else hipLaunchKernelGGL((k_samples<5, 2>), gf, b, 0, c->stream, p, dX, dd);
     } else {
-        hipLaunchKernelGGL((k_cells<4, float>), gc, b, 0, c->stream, p);
+        hipLaunchKernelGGL((k_cells<4, float>), gc, bc, 0, c->stream, p);
         if (axis == 0) hipLaunchKernelGGL((k_samples<4, 0>), gf, b, 0, c->stream, p, dX, dd);
         else hipLaunchKernelGGL((k_samples<4, 1>), gf, b, 0, c->stream, p, dX, dd);
     }

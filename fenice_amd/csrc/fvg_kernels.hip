@@ -67,7 +67,7 @@ __device__ __forceinline__ long long xcd_block() {
 
 __device__ __forceinline__ unsigned mdiv(unsigned n, Magic g) { return (__umulhi(n, g.m) + n) >> g.l; }
 
-// The row and chunk of a row block (a grid of nplanes * ny * nchunk blocks of kRowBlock threads; the row of
+// The row and chunk of a row block (a grid of nplanes * ny * nchunk blocks of rbt threads; the row of
 // (j, k) holds nx elements). Blocks are dispatched round-robin over the 8 XCDs; when ny divides by 8, XCD x takes
 // the rows j in [x ny/8, (x+1) ny/8) of every plane, plane after plane, so the few planes a z-stencil spans of its
 // slab stay in that XCD's L2. Otherwise the rows go in mesh order.
@@ -87,7 +87,7 @@ __device__ __forceinline__ RowBlock row_block(const KP& p) {
     }
     const unsigned r = mdiv(m, p.mg_chunk), chunk = m - r * p.nchunk;
     const unsigned k = mdiv(r, rows), jj = r - k * nrow;
-    return {int(chunk * kRowBlock + threadIdx.x), int(j0 + jj), int(k)};
+    return {int(chunk * p.rbt + threadIdx.x), int(j0 + jj), int(k)};
 }
 
 // Wave-level then block-level maximum of non-negative, non-NaN doubles, then one atomicMax of the bit pattern.

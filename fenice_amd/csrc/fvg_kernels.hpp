@@ -56,9 +56,10 @@ struct KP {
     // multiples and start their interior at PX0, so that every row's interior starts on a cache-line boundary.
     int px, py, phz;
     long long npad;
-    // row blocks (k_cells, k_update, z faces): blocks of kBlock threads over chunks of a row of nx elements;
-    // divisors by multiply-high (Magic): chunks per row, rows per XCD slab (ny / 8), rows per plane (ny)
-    int nchunk, slabs;   // slabs: 1 when ny divides by 8 (each XCD takes a y-slab), else 0 (rows in mesh order)
+    // row blocks (k_cells, k_update, z faces): blocks of rbt threads (nx rounded up to a multiple of 64, at most
+    // kRowBlock) over chunks of a row of nx elements; divisors by multiply-high (Magic): chunks per row, rows per
+    // XCD slab (ny / 8), rows per plane (ny)
+    int rbt, nchunk, slabs;   // slabs: 1 when ny divides by 8 (each XCD takes a y-slab), else 0 (rows in mesh order)
     Magic mg_chunk, mg_slab, mg_ny;
     Magic mg_fi[3], mg_fj[3];   // faces per row and rows per plane of the faces normal to each axis
     // buffers (plane sets)
@@ -80,7 +81,7 @@ struct KP {
     double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };
 
-constexpr int kRowBlock = 256;   // threads of a row block
+constexpr int kRowBlock = 256;   // most threads of a row block
 constexpr int PH = 3;     // halo layers of the padded primitive planes (the WENO5-Z stencil reaches 3 cells out)
 constexpr int PXA = 32;   // row alignment of the padded planes in elements (128 B in FP32, 256 B in FP64)
 constexpr int PX0 = 32;   // element of a padded row that holds i = 0 (the low x halo sits just before it)
