@@ -18,8 +18,8 @@ LIB = os.path.join(HERE, "libfvg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FVG_ARCH", "gfx950")
 
-SOURCES = ["fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip", "fvg_config.cpp", "fvg_checkpoint.cpp",
-           "fvg_npz.cpp"]
+SOURCES = ["fvg_kernels_f64.hip", "fvg_kernels_f32.hip", "fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip",
+           "fvg_config.cpp", "fvg_checkpoint.cpp", "fvg_npz.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-I" + os.path.join(ROOT, "include")]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -48,7 +48,7 @@ def build(verbose: bool = False) -> str:
         objs.append(obj)
         if verbose:
             print(" ".join(cmds[-1]), flush=True)
-    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])

@@ -1,5 +1,5 @@
 // Host side of the C ABI (include/fvg.h): a context owns one device, one stream and the device buffers of one
-// mesh; residual / step / advance enqueue the kernels of fvg_kernels.hip and keep time, dt and the divergence flag
+// mesh; residual / step / advance enqueue the kernels of fvg_kernels_impl.hpp and keep time, dt and the divergence flag
 // on the device. Every HIP call is checked; failures come back as status codes with a message.
 #include <hip/hip_runtime.h>
 

@@ -1,0 +1,35 @@
+// The FP64 instantiations of the residual kernels (the reference's arithmetic, bit for bit).
+#include "fvg_kernels_impl.hpp"
+
+namespace fvg {
+
+#define FVG_INST_T(NV, T)                                                                                            \
+    template __global__ void k_cells<NV, T>(KP);                                                                     \
+    template __global__ void k_faces<NV, 0, MODE_PLAIN, T>(KP);                                                      \
+    template __global__ void k_faces<NV, 1, MODE_PLAIN, T>(KP);                                                      \
+    template __global__ void k_faces<NV, 0, MODE_FIRST_ORDER, T>(KP);                                                \
+    template __global__ void k_faces<NV, 1, MODE_FIRST_ORDER, T>(KP);                                                \
+    template __global__ void k_update<NV, UPD_RESIDUAL, T>(KP, const double*, const double*, double*);              \
+    template __global__ void k_update<NV, UPD_AXPY, T>(KP, const double*, const double*, double*);                  \
+    template __global__ void k_update<NV, UPD_RK2_2, T>(KP, const double*, const double*, double*);                 \
+    template __global__ void k_update<NV, UPD_RK3_2, T>(KP, const double*, const double*, double*);                 \
+    template __global__ void k_update<NV, UPD_RK3_3, T>(KP, const double*, const double*, double*);
+#define FVG_ZUPD(NV, M, T)                                                                                 \
+    template __global__ void k_zupdate<NV, M, UPD_RK2_2, T>(KP, const double*, const double*, double*); \
+    template __global__ void k_zupdate<NV, M, UPD_RK3_2, T>(KP, const double*, const double*, double*); \
+    template __global__ void k_zupdate<NV, M, UPD_RK3_3, T>(KP, const double*, const double*, double*);
+FVG_INST_T(4, double)
+FVG_INST_T(5, double)
+template __global__ void k_faces<5, 2, MODE_PLAIN, double>(KP);
+template __global__ void k_faces<5, 2, MODE_FIRST_ORDER, double>(KP);
+template __global__ void k_faces<4, 0, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<4, 1, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<5, 0, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<5, 1, MODE_HANCOCK, double>(KP);
+template __global__ void k_faces<5, 2, MODE_HANCOCK, double>(KP);
+FVG_ZUPD(5, MODE_PLAIN, double)
+FVG_ZUPD(5, MODE_FIRST_ORDER, double)
+FVG_ZUPD(4, MODE_PLAIN, double)
+FVG_ZUPD(4, MODE_FIRST_ORDER, double)
+
+}   // namespace fvg
