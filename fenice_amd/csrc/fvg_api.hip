@@ -65,6 +65,7 @@ struct fvg_ctx {
     bool fp32 = false;
     double *tot_part = nullptr, *tot_out = nullptr;
     DevState* st = nullptr;
+    unsigned long long* spd = nullptr;   // wave-speed slots (KP::spd)
     DevState* st_host = nullptr;   // pinned mirror for the read-back
     KP base{};
     bool lagged_positive = false;
@@ -241,7 +242,7 @@ int enqueue_indicator(fvg_ctx* c, int op) {
 
 int enqueue_control(fvg_ctx* c, int op, int arg) {
     const KP p = c->base;
-    return launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, c->stream, p, op, arg); });
+    return launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(kSpeedSlots), 0, c->stream, p, op, arg); });
 }
 
 template <int NV, typename T>
@@ -441,7 +442,8 @@ void free_ctx(fvg_ctx* c) {
     if (c->graph) hipGraphExecDestroy(c->graph);
     hipFree(c->mlp);
     hipFree(c->ind);
-    void* bufs[] = {c->U, c->Ut, c->Rb, c->W, c->G, c->HS, c->Fc[0], c->Fc[1], c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out};
+    void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,       c->HS,   c->Fc[0], c->Fc[1],
+                    c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->st) hipFree(c->st);
@@ -681,6 +683,9 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         }
     }
     if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
+    const size_t spd_bytes = sizeof(unsigned long long) * 2 * kSpeedSlots * kSlotStride;
+    if (!rc && (hipMalloc(&c->spd, spd_bytes) != hipSuccess || hipMemset(c->spd, 0, spd_bytes) != hipSuccess))
+        rc = set_err(c, FVG_ENOMEM, "hipMalloc of the wave-speed slots");
     if (!rc && hipHostMalloc(&c->st_host, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipHostMalloc of the state mirror");
     if (rc) {
         const std::string m = c->err;
@@ -702,6 +707,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         p.nfa[d] = c->nfa[d];
     }
     p.st = c->st;
+    p.spd = c->spd;
     p.mlp = c->mlp;
     p.ind = c->ind;
     p.ind_audit = cfg.indicator_audit;
@@ -953,7 +959,8 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
         hipFree(dX);
         return set_err(c, FVG_ENOMEM, "hipMalloc of the samples");
     }
-    const KP p = make_kp(c, c->U);
+    KP p = make_kp(c, c->U);
+    p.viscous = 0;   // only the primitives are needed: no gradients, and no speed maxima left behind in the slots
     const dim3 gc(cells_grid(c)), bc(cells_block(c)), gf((unsigned)((nfaces + kBlock - 1) / kBlock)), b(kBlock);
     if (c->NV == 5) {
         hipLaunchKernelGGL((k_cells<5, float>), gc, bc, 0, c->stream, p);

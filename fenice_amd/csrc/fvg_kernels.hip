@@ -32,9 +32,31 @@ __device__ __forceinline__ void end_step(DevState* s) {
 }
 
 // op: the CTL_* step; arg: the time scheme (CTL_STAGE1_DT, CTL_END_RK) or the stage number (CTL_STAGE_S).
+// Launched with kSpeedSlots threads: each takes one slot of each speed maximum, clears it, and the wavefront's
+// maxima become s_conv / s_visc; the rest runs on one thread.
 __global__ void k_control(KP p, int op, int arg) {
     const int scheme = arg;
     DevState* s = p.st;
+    if (!s->done) {
+        const int l = threadIdx.x;
+        unsigned long long* cs = p.spd + (size_t)(SPD_CONV * kSpeedSlots + l) * kSlotStride;
+        unsigned long long* vs = p.spd + (size_t)(SPD_VISC * kSpeedSlots + l) * kSlotStride;
+        unsigned long long c = *cs, v = *vs;
+        *cs = 0;
+        *vs = 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {   // bit patterns of non-negative doubles order as integers
+            const unsigned long long oc = __shfl_xor(c, o, 64), ov = __shfl_xor(v, o, 64);
+            c = oc > c ? oc : c;
+            v = ov > v ? ov : v;
+        }
+        __syncthreads();   // every lane has read s->done before thread 0 may set it
+        if (l == 0) {
+            s->s_conv = c;
+            s->s_visc = v;
+        }
+    }
+    if (threadIdx.x != 0) return;
     if (op == CTL_BEGIN_STEP) {
         if (s->done) return;
         if (!(s->t < s->t_end) || (s->max_steps > 0 && s->steps >= s->max_steps)) {

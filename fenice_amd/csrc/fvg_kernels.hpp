@@ -72,6 +72,10 @@ struct KP {
     void* Fv[3];        // viscous face flux per axis (3-D: slots 1..4; 2-D: slots 1..3)
     long long nfa[3];   // faces per axis (plane stride of Fc / Fv)
     DevState* st;
+    // the block maxima of the wave speeds land in kSpeedSlots slots per quantity (convective, viscous), chosen by
+    // block index and kSlotStride elements apart, so that the atomics of a pass spread over L2 channels;
+    // k_control reduces and clears them
+    unsigned long long* spd;
     // learned troubled-face indicator (FP32 fluxes, WENO5-Z)
     const MlpDev* mlp;
     IndState* ind;
@@ -81,6 +85,8 @@ struct KP {
     double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };
 
+constexpr int kSpeedSlots = 64, kSlotStride = 32;   // 64 slots per quantity, 256 B apart
+constexpr int SPD_CONV = 0, SPD_VISC = 1;
 constexpr int kRowBlock = 256;   // most threads of a row block
 constexpr int PH = 3;     // halo layers of the padded primitive planes (the WENO5-Z stencil reaches 3 cells out)
 constexpr int PXA = 32;   // row alignment of the padded planes in elements (128 B in FP32, 256 B in FP64)

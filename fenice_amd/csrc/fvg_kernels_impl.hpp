@@ -112,6 +112,10 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
         if (m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
     }
 }
+// the block maximum into this block's slot of quantity q (SPD_CONV, SPD_VISC)
+__device__ __forceinline__ void block_max_slot(const KP& p, double v, int q) {
+    block_max_to(v, p.spd + ((size_t)q * kSpeedSlots + blockIdx.x % kSpeedSlots) * kSlotStride);
+}
 
 // ------------------------------------------------------------------------------------------------------------
 // Stencil access
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, i, j, k, w));
         }
     }
-    if (p.viscous) block_max_to(local, &p.st->s_visc);
+    if (p.viscous) block_max_slot(p, local, SPD_VISC);
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -770,7 +774,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (double(sm) > local) local = double(sm);
         store_face<NV, D, T>(p, f, i, j, k, t, n, F);
     }
-    block_max_to(local, &p.st->s_conv);
+    block_max_slot(p, local, SPD_CONV);
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -861,7 +865,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (double(sm) > local) local = double(sm);
         store_face<NV, D, float>(p, f, i, j, k, t, n, F);
     }
-    block_max_to(local, &p.st->s_conv);
+    block_max_slot(p, local, SPD_CONV);
 }
 
 // Audit of the indicator on the stage input of an audit step: the network's decision against the label (WENO5-Z
@@ -1137,7 +1141,7 @@ __global__ __launch_bounds__(1024) void k_zupdate(KP p, const double* __restrict
         }
         combine<NV, UPD>(p, c, R, U0, Ut, out);
     }
-    block_max_to(local, &p.st->s_conv);
+    block_max_slot(p, local, SPD_CONV);
 }
 
 }   // namespace fvg
