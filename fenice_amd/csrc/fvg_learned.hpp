@@ -51,24 +51,32 @@ constexpr int kAuditSlots = 64;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bfloat16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ unsigned bf16_bits(float x) {
-    const unsigned u = __float_as_uint(x);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> two bfloat16 in one dword (a in the low half), rounded to nearest even (gfx950's
+// v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 __device__ __forceinline__ unsigned long long pack4(float a, float b, float c, float d) {
-    return (unsigned long long)bf16_bits(a) | ((unsigned long long)bf16_bits(b) << 16) | ((unsigned long long)bf16_bits(c) << 32) |
-           ((unsigned long long)bf16_bits(d) << 48);
+    return (unsigned long long)pack2(a, b) | ((unsigned long long)pack2(c, d) << 32);
 }
 __device__ __forceinline__ bf16x8 as_frag(unsigned long long lo, unsigned long long hi) {
     Frag8 f{lo, hi};
     return __builtin_bit_cast(bf16x8, f);
 }
-__device__ __forceinline__ unsigned long long xor32(unsigned long long v) {
-    const int lo = __shfl_xor((int)(unsigned)v, 32, 64);
-    const int hi = __shfl_xor((int)(unsigned)(v >> 32), 32, 64);
-    return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+// the value of lane l ^ 32 (gfx950's v_permlane32_swap: of the two copies of v, the first gets lanes 0-31 of the
+// second in its lanes 32-63, the second gets lanes 32-63 of the first in its lanes 0-31)
+__device__ __forceinline__ unsigned xor32(unsigned v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
 }
+__device__ __forceinline__ unsigned long long xor32(unsigned long long v) {
+    return (unsigned long long)xor32((unsigned)v) | ((unsigned long long)xor32((unsigned)(v >> 32)) << 32);
+}
+__device__ __forceinline__ float xor32(float v) { return __uint_as_float(xor32(__float_as_uint(v))); }
 
 // linear upwind-5 face value (weno5z with the linear weights)
 template <typename T> __device__ __forceinline__ T upwind5(T c0, T c1, T c2, T c3, T c4) {
@@ -190,7 +198,7 @@ __device__ __forceinline__ float mlp_logit_wave(const MlpDev* __restrict__ m, co
         float part = 0.0f;
 #pragma unroll
         for (int e = 0; e < 16; ++e) part += m->w3h[half][e] * fmaxf(acc2[e], 0.0f);
-        const float z = part + __shfl_xor(part, 32, 64) + m->b3;
+        const float z = part + xor32(part) + m->b3;
         if (own) out = z;
     }
     return out;

@@ -33,7 +33,7 @@ def const_params(b3):
 
 
 def test_matrix_core_inference_matches_cpu():
-    # every lane layout of the two chained 32x32x8 bfloat16 products: random weights and features, 1000 faces
+    # every lane layout of the chained 32x32x16 bfloat16 products: random weights and features, 1000 faces
     say("mlp logits")
     rng = np.random.default_rng(3)
     X = rng.uniform(-1, 1, (1000, 15)).astype(np.float32)
