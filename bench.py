@@ -30,7 +30,7 @@ BYTES_PER_CELL_STAGE_RK3_3D = 8.0 * 5 * (2 + 3 + 3) / 3.0   # 106.7 B: stage inp
 # HBM bytes of one stage of the 256^3 workload (k_cells + k_faces x/y/z + k_update, the update averaged over the
 # three SSP-RK3 stages), from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB
 # device copy reads in the same runs); the runs and per-kernel figures are in profiles/tgv256_weno5_rk3_<flux>.md
-MEASURED_STAGE_TRAFFIC = {("fp64", 256): 16.594e9, ("fp32", 256): 7.662e9}
+MEASURED_STAGE_TRAFFIC = {("fp64", 256): 16.776e9, ("fp32", 256): 7.678e9}
 
 from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
 from fenice_amd.workloads import taylor_green_keys as run_keys  # noqa: E402
