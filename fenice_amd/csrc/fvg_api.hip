@@ -20,13 +20,13 @@ template <int NV, typename T> __global__ void k_cells(KP);
 template <int NV> __global__ void k_hancock(KP);
 template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
-template <int NV> __global__ void k_totals_partial(const double*, long long, double*);
+template <int NV, int WHAT> __global__ void k_totals_partial(const double*, long long, double*);
 template <int NV, int D> __global__ void k_faces_learned(KP);
 template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
 template <int NV, int D> __global__ void k_audit(KP);
 template <int NV, int D> __global__ void k_samples(KP, float*, float*);
 __global__ void k_indicator_control(KP, int);
-__global__ void k_totals_final(const double*, int, long long, double*);
+__global__ void k_totals_final(const double*, int, long long, int, double*);
 __global__ void k_control(KP, int, int);
 __global__ void k_riemann_batch(RiemannParams, const double*, const double*, const int*, long long, double*, double*);
 }   // namespace fvg
@@ -901,17 +901,33 @@ int fvg_checkpoint_read(fvg_ctx* c, const char* path, double* t, int* step) {
     return FVG_OK;
 }
 
-int fvg_conserved_totals(fvg_ctx* c, double out[3]) {
-    if (!c || !out) return set_err(c, FVG_EINVAL, "null argument");
-    HIPCHK(c, hipSetDevice(c->device));
+// the three sums of a plane set (what 0: conserved totals of a state, 1: residual sums of squares) into out[3]
+static int plane_sums(fvg_ctx* c, int what, const double* planes, double out[3]) {
     const int nb = kTotalsBlocks;
-    if (c->NV == 5) hipLaunchKernelGGL(k_totals_partial<5>, dim3(nb), dim3(256), 0, c->stream, c->U, c->nc, c->tot_part);
-    else hipLaunchKernelGGL(k_totals_partial<4>, dim3(nb), dim3(256), 0, c->stream, c->U, c->nc, c->tot_part);
-    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, c->stream, c->tot_part, nb, c->nc, c->tot_out);
+    const dim3 g(nb), b(256);
+    if (c->NV == 5 && what == 0) hipLaunchKernelGGL((k_totals_partial<5, 0>), g, b, 0, c->stream, planes, c->nc, c->tot_part);
+    else if (c->NV == 5) hipLaunchKernelGGL((k_totals_partial<5, 1>), g, b, 0, c->stream, planes, c->nc, c->tot_part);
+    else if (what == 0) hipLaunchKernelGGL((k_totals_partial<4, 0>), g, b, 0, c->stream, planes, c->nc, c->tot_part);
+    else hipLaunchKernelGGL((k_totals_partial<4, 1>), g, b, 0, c->stream, planes, c->nc, c->tot_part);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, c->stream, c->tot_part, nb, c->nc, what, c->tot_out);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(out, c->tot_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVG_OK;
+}
+
+int fvg_conserved_totals(fvg_ctx* c, double out[3]) {
+    if (!c || !out) return set_err(c, FVG_EINVAL, "null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    return plane_sums(c, 0, c->U, out);
+}
+
+int fvg_monitor(fvg_ctx* c, double out[6]) {
+    if (!c || !out) return set_err(c, FVG_EINVAL, "null argument");
+    int rc;
+    if ((rc = fvg_residual(c, 0.0, nullptr, 5, nullptr))) return rc;   // the plain residual of the state, into Rb
+    if ((rc = plane_sums(c, 0, c->U, out))) return rc;
+    return plane_sums(c, 1, c->Rb, out + 3);
 }
 
 int fvg_indicator_set(fvg_ctx* c, const float* params, int nparam, float threshold) {

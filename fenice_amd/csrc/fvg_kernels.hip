@@ -137,9 +137,12 @@ __global__ void k_indicator_control(KP p, int op) {
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Conserved totals: a fixed-order FP64 tree (per-block partial sums, then one block over the partials).
+// Conserved totals and residual norms: a fixed-order FP64 tree (per-block partial sums, then one block over the
+// partials). WHAT 0: mass, kinetic energy and total energy of a state (volume_averages, SimulationRun.cpp:45-64);
+// WHAT 1: the sums of squares of a residual per quantity, rho, momentum and energy (add_residual,
+// SimulationRun.cpp:121-126).
 // ------------------------------------------------------------------------------------------------------------
-template <int NV>
+template <int NV, int WHAT>
 __global__ __launch_bounds__(256) void k_totals_partial(const double* __restrict__ U, long long nc, double* __restrict__ part) {
     __shared__ double sh[3][256];
     const long long per = (nc + gridDim.x - 1) / gridDim.x;
@@ -148,9 +151,15 @@ __global__ __launch_bounds__(256) void k_totals_partial(const double* __restrict
     for (long long c = b0 + threadIdx.x; c < b1; c += blockDim.x) {
         const double r = U[c], m1 = U[nc + c], m2 = U[2 * nc + c], E = U[3 * nc + c];
         const double m4 = NV == 5 ? U[(NV - 1) * nc + c] : 0.0;
-        a0 += r;
-        a1 += 0.5 * (m1 * m1 + m2 * m2 + m4 * m4) / r;
-        a2 += E;
+        if constexpr (WHAT == 0) {
+            a0 += r;
+            a1 += 0.5 * (m1 * m1 + m2 * m2 + m4 * m4) / r;
+            a2 += E;
+        } else {
+            a0 += r * r * 1.0;
+            a1 += (m1 * m1 + m2 * m2 + m4 * m4) * 1.0;
+            a2 += E * E * 1.0;
+        }
     }
     sh[0][threadIdx.x] = a0;
     sh[1][threadIdx.x] = a1;
@@ -164,12 +173,13 @@ __global__ __launch_bounds__(256) void k_totals_partial(const double* __restrict
     if (threadIdx.x == 0)
         for (int q = 0; q < 3; ++q) part[3 * blockIdx.x + q] = sh[q][0];
 }
-__global__ void k_totals_final(const double* __restrict__ part, int nb, long long nc, double* __restrict__ out) {
+// means (norm 0), or root mean squares (norm 1: residual_norms, SimulationRun.cpp:128-131, every weight 1)
+__global__ void k_totals_final(const double* __restrict__ part, int nb, long long nc, int norm, double* __restrict__ out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double s[3] = {0.0, 0.0, 0.0};
     for (int b = 0; b < nb; ++b)
         for (int q = 0; q < 3; ++q) s[q] += part[3 * b + q];
-    for (int q = 0; q < 3; ++q) out[q] = s[q] / (double)nc;
+    for (int q = 0; q < 3; ++q) out[q] = norm ? (nc > 0 ? sqrt(s[q] / (double)nc) : 0.0) : s[q] / (double)nc;
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -210,7 +220,9 @@ template __global__ void k_samples<4, 1>(KP, float*, float*);
 template __global__ void k_samples<5, 0>(KP, float*, float*);
 template __global__ void k_samples<5, 1>(KP, float*, float*);
 template __global__ void k_samples<5, 2>(KP, float*, float*);
-template __global__ void k_totals_partial<4>(const double*, long long, double*);
-template __global__ void k_totals_partial<5>(const double*, long long, double*);
+template __global__ void k_totals_partial<4, 0>(const double*, long long, double*);
+template __global__ void k_totals_partial<5, 0>(const double*, long long, double*);
+template __global__ void k_totals_partial<4, 1>(const double*, long long, double*);
+template __global__ void k_totals_partial<5, 1>(const double*, long long, double*);
 
 }   // namespace fvg

@@ -31,7 +31,7 @@ EXPORTS = [
     "fvg_batch_run", "fvg_batch_set_block", "fvg_batch_get_block", "fvg_batch_tune",
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
     "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
-    "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read",
+    "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read", "fvg_monitor",
 ]
 
 MLP_NPARAM = 1601
@@ -133,6 +133,7 @@ def lib():
         L.fvg_checkpoint_load.argtypes = [P, C.c_char_p, _D, C.c_size_t, C.c_int, _D, _I, _D, C.c_char_p, C.c_size_t]
         L.fvg_checkpoint_write.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int]
         L.fvg_checkpoint_read.argtypes = [C.c_void_p, C.c_char_p, _D, _I]
+        L.fvg_monitor.argtypes = [C.c_void_p, _D]
         _lib = L
     return _lib
 
@@ -257,6 +258,13 @@ class Solver:
         t, n = C.c_double(), C.c_int()
         self._c(lib().fvg_checkpoint_read(self._h, os.fsencode(path), C.byref(t), C.byref(n)))
         return t.value, n.value
+
+    def monitor(self):
+        """(mass, kinetic_energy, total_energy, res_rho, res_mom, res_E) of the state: a row of the reference's monitor
+        file (fvg_monitor)."""
+        out = np.empty(6)
+        self._c(lib().fvg_monitor(self._h, _dp(out)))
+        return out
 
     def conserved_totals(self):
         out = np.empty(3)

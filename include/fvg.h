@@ -172,6 +172,14 @@ int fvg_checkpoint_read(fvg_ctx* ctx, const char* path, double* t, int* step);
  * sequential sum to round-off, not bit for bit. */
 int fvg_conserved_totals(fvg_ctx* ctx, double out[3]);
 
+/* The numbers of a row of the reference's monitor file (`<prefix>_monitor.csv`: step, t, mass, kinetic_energy,
+ * total_energy, res_rho, res_mom, res_E; Simulation::monitor_row and its uniform-mesh caller,
+ * SimulationRun.cpp:76-131, 413-424): out[0..2] the volume averages of the state (fvg_conserved_totals), out[3..5] the
+ * root mean squares over the cells of the plain residual of the state for density, momentum (all components) and
+ * energy. The residual is the reference's bits; its sums are a fixed-order tree, so the norms match the reference's
+ * sequential sums to round-off. */
+int fvg_monitor(fvg_ctx* ctx, double out[6]);
+
 /* Timing of the kernels of the last fvg_advance / fvg_step calls, measured with HIP events on the
  * context's stream: name[k] (static strings), total milliseconds and launch count per kernel family.
  * Returns the number of families written (at most cap). */

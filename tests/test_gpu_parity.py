@@ -260,3 +260,31 @@ def test_fused_stages_match_separate_passes(oracle, case):
         Uo = U.copy()
         oracle.run(cfg, Uo, 100.0, max_steps=4)
         assert same_bits(out[0][0], Uo)
+
+
+@pytest.mark.parametrize("kind", ["tgv3d", "vortex2d"])
+def test_monitor_row_matches_the_oracle(oracle, kind):
+    # the numbers of a monitor row (SimulationRun.cpp:413-424): volume averages of the state and the RMS norms of its
+    # plain residual, against the same sums over the oracle's residual (bit-identical to the device's); the sums
+    # differ only in order
+    from fenice_amd.workloads import taylor_green, taylor_green_keys
+    from cases import VORTEX_HANCOCK_256, vortex_state
+    if kind == "tgv3d":
+        keys, U = taylor_green_keys(20), taylor_green(20)
+    else:
+        keys, U = dict(VORTEX_HANCOCK_256, nx="48", ny="48"), vortex_state(48)
+    cfg = fvg.config_from_keys(keys)
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        s.advance(0.0, 1e9, max_steps=3)
+        Ud = s.download()
+        row = s.monitor()
+    U5 = np.zeros((Ud.shape[0], 5))
+    U5[:, :Ud.shape[1]] = Ud
+    R, _ = oracle.residual(cfg, U5)
+    n = U5.shape[0]
+    want = [U5[:, 0].sum() / n, (0.5 * (U5[:, 1] ** 2 + U5[:, 2] ** 2 + U5[:, 4] ** 2) / U5[:, 0]).sum() / n, U5[:, 3].sum() / n,
+            np.sqrt((R[:, 0] ** 2).sum() / n), np.sqrt((R[:, 1] ** 2 + R[:, 2] ** 2 + R[:, 4] ** 2).sum() / n),
+            np.sqrt((R[:, 3] ** 2).sum() / n)]
+    assert np.all(np.abs(row - want) <= 1e-12 * np.abs(want)), (row, want)
+    assert row[3] > 0 and row[4] > 0 and row[5] > 0
