@@ -103,6 +103,16 @@ def matrix():
                                                              use_viscous="true", mu="0.01", **walls3), "smooth"))
         out.append((f"3d-walls-hancock-bo{bo}", base_keys(8, 9, 7, periodic=False, boundary_order=bo, time_scheme="MUSCL-Hancock",
                                                           **walls3), "shock"))
+    # thin meshes: axes as deep as the stencil's halo (periodic wrap or ghost slab of exactly Dh cells) or shallower
+    # (no halo: edge copies), where every halo layer comes from a cell next to the other side or from the edge cell
+    out.append(("3d-thin-weno5-periodic-n3", base_keys(5, 3, 3, reconstruction="weno5"), "shock"))
+    out.append(("3d-thin-weno5-periodic-n2", base_keys(6, 4, 2, reconstruction="weno5"), "shock"))
+    out.append(("3d-thin-visc-weno5-n3", base_keys(4, 3, 5, reconstruction="weno5", use_viscous="true", mu="0.01"), "smooth"))
+    out.append(("3d-thin-muscl-walls-bo2-n2", base_keys(5, 2, 3, periodic=False, boundary_order=2, **walls3), "shock"))
+    out.append(("2d-thin-weno5-walls-bo2-n3", base_keys(7, 3, periodic=False, boundary_order=2, reconstruction="weno5", **walls), "shock"))
+    out.append(("2d-thin-weno3-periodic-n2", base_keys(9, 2, reconstruction="weno3"), "shock"))
+    out.append(("2d-thin-hancock-walls-bo2-n2", base_keys(6, 2, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **walls),
+                "shock"))
     return out
 
 
