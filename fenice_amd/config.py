@@ -18,6 +18,7 @@ TS_AUTO, TS_EULER, TS_SSPRK2, TS_SSPRK3, TS_HANCOCK = -1, 0, 1, 2, 3
 EOS_IDEAL, EOS_STIFFENED = 0, 1
 FLUX_FP64, FLUX_FP32 = 0, 1
 INDICATOR_NONE, INDICATOR_LEARNED = 0, 1
+QUAD_MIDPOINT, QUAD_FOURTH = 0, 1
 
 # status codes
 OK, EINVAL, ELOGIC, EDIVERGED, EHIP, ENOMEM, EUNSUPPORTED, EIO = range(8)
@@ -38,6 +39,7 @@ class Config(C.Structure):
         ("flux_precision", C.c_int),
         ("indicator", C.c_int), ("indicator_audit", C.c_int), ("indicator_label", C.c_double),
         ("indicator_fn_limit", C.c_double), ("indicator_fn_floor", C.c_double),
+        ("flux_quadrature", C.c_int),
     ]
 
     @staticmethod
@@ -59,6 +61,7 @@ class Config(C.Structure):
         c.flux_precision = FLUX_FP64
         c.indicator, c.indicator_audit, c.indicator_label, c.indicator_fn_limit = INDICATOR_NONE, 50, 0.05, 0.01
         c.indicator_fn_floor = 1e-4
+        c.flux_quadrature = QUAD_MIDPOINT
         return c
 
     @staticmethod

@@ -87,6 +87,7 @@ void fvg_config_defaults(fvg_config* c) {
     c->indicator_label = 0.05;
     c->indicator_fn_limit = 0.01;
     c->indicator_fn_floor = 1e-4;
+    c->flux_quadrature = FVG_QUAD_MIDPOINT;
 }
 
 int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* err, size_t n) {
@@ -178,6 +179,11 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
     if (key == "indicator_label") return num(c->indicator_label);
     if (key == "indicator_fn_limit") return num(c->indicator_fn_limit);
     if (key == "indicator_fn_floor") return num(c->indicator_fn_floor);
+    if (key == "flux_quadrature") {   // Params.cpp:348, 448-449
+        if (lv == "midpoint") return (c->flux_quadrature = FVG_QUAD_MIDPOINT), FVG_OK;
+        if (lv == "fourth") return (c->flux_quadrature = FVG_QUAD_FOURTH), FVG_OK;
+        return fail(err, n, "flux_quadrature must be midpoint or fourth; got '" + lv + "'");
+    }
     static const char* bc_keys[6] = {"bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax", "bc_zmin", "bc_zmax"};
     for (int s = 0; s < 6; ++s) {
         if (key != bc_keys[s]) continue;
@@ -216,6 +222,10 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
                                 " is not used with SSP Runge-Kutta 2 (use SSP Runge-Kutta 3)");
     if (c->time_scheme == FVG_TS_HANCOCK && c->reconstruction != FVG_RECON_MUSCL)
         return fail(err, n, "time_scheme = MUSCL-Hancock needs reconstruction = muscl");
+    if (c->flux_quadrature != FVG_QUAD_MIDPOINT && c->flux_quadrature != FVG_QUAD_FOURTH)
+        return fail(err, n, "flux_quadrature must be midpoint or fourth");
+    if (c->flux_quadrature == FVG_QUAD_FOURTH && c->reconstruction != FVG_RECON_WENO3 && c->reconstruction != FVG_RECON_WENO5)
+        return fail(err, n, "flux_quadrature = fourth needs reconstruction = weno3 or weno5");
     if (c->boundary_order != 1 && c->boundary_order != 2) return fail(err, n, "boundary_order must be 1 or 2");
     if (c->indicator == FVG_INDICATOR_LEARNED &&
         (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))

@@ -68,6 +68,11 @@ enum fvg_flux_precision { FVG_FLUX_FP64 = 0, FVG_FLUX_FP32 = 1 };
  * smooth flow, e.g. where a velocity component crosses zero, from switching it off). */
 enum fvg_indicator { FVG_INDICATOR_NONE = 0, FVG_INDICATOR_LEARNED = 1 };
 
+/* Flux quadrature over a face (flux_quadrature, Params.cpp:148, 448-455). FOURTH: WENO3-Z / WENO5-Z of the conserved
+ * averages, point values and a fourth-order transverse correction of the face fluxes (Schemes.cpp:1954-2134); needs
+ * reconstruction = weno3 or weno5. */
+enum fvg_flux_quadrature { FVG_QUAD_MIDPOINT = 0, FVG_QUAD_FOURTH = 1 };
+
 /* The keys of the path (cpp/src/Params.cpp:98-223 and SimParams::from), typed. Defaults: fvg_config_defaults. */
 typedef struct fvg_config {
     int nx, ny, nz;              /* cells; nz = 1 is a 2-D mesh */
@@ -94,6 +99,7 @@ typedef struct fvg_config {
     double indicator_label;      /* weight departure above which a face is troubled (default 0.05) */
     double indicator_fn_limit;   /* fallback limit of the missed fraction of troubled faces (default 0.01) */
     double indicator_fn_floor;   /* fallback limit of the missed fraction of all audited faces (default 1e-4) */
+    int flux_quadrature;         /* enum fvg_flux_quadrature (default MIDPOINT) */
 } fvg_config;
 
 /* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,

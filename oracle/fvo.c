@@ -522,9 +522,10 @@ static int halo_state(const fvg_config* c, const Mesh* m, const Eos* e, const do
 
 static inline int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
-/* pad_primitives (Schemes.cpp:915-969): clamp-replicated primitive state, with the halo slabs in place. */
+/* pad_primitives (Schemes.cpp:915-969): clamp-replicated primitive state (prim = 0: the conservative state as it
+ * is), with the halo slabs in place. */
 static int pad_primitives(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, int pad, const int has[6],
-                          const int wrap[3], Padded* P) {
+                          const int wrap[3], Padded* P, int prim) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
     P->pad = pad;
     P->zp = m->is3d ? pad : 0;
@@ -553,7 +554,8 @@ static int pad_primitives(const fvg_config* c, const Mesh* m, const Eos* e, cons
                     src = Uc;
                 }
             }
-            c2p(e, src, W, NULL);
+            if (prim) c2p(e, src, W, NULL);
+            else memcpy(W, src, sizeof W);
             const size_t p = ((size_t)kk * P->PY + jj) * P->PX + ii;
             for (int q = 0; q < NV; ++q) P->W[q * P->plane + p] = W[q];
         }
@@ -620,6 +622,268 @@ static void reconstruct_faces(const fvg_config* c, const Mesh* m, const Eos* e, 
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* flux_quadrature = fourth (Schemes.cpp:1210-1349, 1954-2134)                                      */
+/* ------------------------------------------------------------------------------------------------ */
+/* p > 0 and rho > 0 without a division: g1 (E rho - |m|^2 / 2) > gp rho (Schemes.cpp:1241-1243, 1967-1968) */
+static inline double cons_m2(const double* u) { return u[1] * u[1] + u[2] * u[2] + u[4] * u[4]; }
+static inline int cons_positive(double g1, double gp, const double* u) {
+    return u[0] > 0.0 && g1 * (u[3] * u[0] - 0.5 * cons_m2(u)) > gp * u[0];
+}
+static inline double cons_pressure(double g1, double gp, const double* u) { return g1 * (u[3] - 0.5 * cons_m2(u) / u[0]) - gp; }
+
+/* MusclReconstruction::reconstruct_conserved / ConservedLines::line (Schemes.cpp:1210-1349): WENO3-Z (four cells)
+ * or WENO5-Z (six cells) of the conservative averages at every face with a full stencil; a face whose two states
+ * are not positive, or whose density or pressure leaves the range of its stencil cells widened by the range's own
+ * width, takes the two cells next to it. The other faces keep their first-order states as they are. */
+static void reconstruct_conserved(const fvg_config* c, const Mesh* m, const Eos* e, const Padded* P, const int has[6], double* UL,
+                                  double* UR) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const int five = c->reconstruction == FVG_RECON_WENO5;
+    const double we = c->weno_epsilon;
+    const double g1 = e->gamma - 1.0, gp = e->stiff ? e->gamma * e->p_inf : 0.0;
+    const int pad = P->pad, left = five ? 2 : 1, ncell = five ? 6 : 4;
+    for (int axis = 0; axis < (m->is3d ? 3 : 2); ++axis) {
+        const int n_a = axis == 0 ? nx : (axis == 1 ? ny : nz);
+        const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)P->PX : (size_t)P->PX * P->PY);
+        const int ni = axis == 0 ? nx + 1 : nx, nj = axis == 1 ? ny + 1 : ny, nk = axis == 2 ? nz + 1 : nz;
+        const int lo = has[2 * axis] != 0, hi = has[2 * axis + 1] != 0;
+#pragma omp parallel for schedule(static) num_threads(NT)
+        for (int kj = 0; kj < nk * nj; ++kj) {
+            const int j = kj % nj, k = kj / nj;
+            for (int i = 0; i < ni; ++i) {
+                const int t = axis == 0 ? i : (axis == 1 ? j : k);
+                if (!((t > 0 && t < n_a) || (t == 0 && lo) || (t == n_a && hi))) continue;
+                const long long f = axis == 0 ? XF(m, i, j, k) : (axis == 1 ? YF(m, i, j, k) : ZF(m, i, j, k));
+                const int pi = axis == 0 ? t : i + pad, pj = axis == 1 ? t : j + pad, pk = axis == 2 ? t : k + P->zp;
+                const size_t base = ((size_t)pk * P->PY + pj) * P->PX + pi;
+                double* ul = UL + NV * f;
+                double* ur = UR + NV * f;
+                for (int q = 0; q < NV; ++q) {
+                    const double* cq = P->W + q * P->plane + base;
+                    if (five) {
+                        ul[q] = weno5z(cq[0], cq[stride], cq[2 * stride], cq[3 * stride], cq[4 * stride], we);
+                        ur[q] = weno5z(cq[5 * stride], cq[4 * stride], cq[3 * stride], cq[2 * stride], cq[stride], we);
+                    } else {
+                        ul[q] = weno3z(cq[0], cq[stride], cq[2 * stride], we);
+                        ur[q] = weno3z(cq[3 * stride], cq[2 * stride], cq[stride], we);
+                    }
+                }
+                int ok = cons_positive(g1, gp, ul) && cons_positive(g1, gp, ur);
+                if (ok) {
+                    double rlo = 1e300, rhi = -1e300, plo = 1e300, phi = -1e300;
+                    for (int s = 0; s < ncell; ++s) {
+                        double u[NV];
+                        for (int q = 0; q < NV; ++q) u[q] = P->W[q * P->plane + base + s * stride];
+                        const double pc = cons_pressure(g1, gp, u);
+                        rlo = smin(rlo, u[0]);
+                        rhi = smax(rhi, u[0]);
+                        plo = smin(plo, pc);
+                        phi = smax(phi, pc);
+                    }
+                    const double rw = rhi - rlo, pw = phi - plo;
+                    rlo -= rw;
+                    rhi += rw;
+                    plo -= pw;
+                    phi += pw;
+                    const double* sts[2] = {ul, ur};
+                    for (int h = 0; h < 2; ++h) {
+                        const double pf = cons_pressure(g1, gp, sts[h]);
+                        ok = ok && sts[h][0] >= rlo && sts[h][0] <= rhi && pf >= plo && pf <= phi;
+                    }
+                }
+                if (ok) continue;
+                for (int q = 0; q < NV; ++q) {
+                    ul[q] = P->W[q * P->plane + base + left * stride];
+                    ur[q] = P->W[q * P->plane + base + (left + 1) * stride];
+                }
+            }
+        }
+    }
+}
+
+/* fourth_order_fluxes (Schemes.cpp:1954-2134). Per face family a (faces normal to axis a), on the lattice
+ * c = (i, j, k) of the family:
+ *   good: a full stencil, both states positive, |pR - pL| <= shock_blend * min(pL, pR) (when shock_blend > 0);
+ *   point values A = U - (0 + sum over transverse axes of (U[+1] - 2 U + U[-1])) / 24 where the face and its
+ *     transverse neighbours are good and A is positive (pt); the neighbours wrap across a periodic axis;
+ *   full: pt and every transverse neighbour pt;
+ *   Fp = the Riemann flux of the point values (of the averages where there are none); its largest wave speed is
+ *     the residual's;
+ *   F = Fp + (0 + sum of (Fp[+1] - 2 Fp + Fp[-1])) / 24 where full, the flux of the averages where pt only, else Fp. */
+static int fourth_order_fluxes(const fvg_config* c, const Mesh* m, const Riemann* rs, const double* UL, const double* UR, double* F,
+                               double* S_out) {
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const int dims[3] = {nx, ny, nz};
+    const int nax = m->is3d ? 3 : 2;
+    const long long nf = m->nf;
+    const double sb = rs->blend;
+    const double g1 = rs->eos.gamma - 1.0, gp = rs->eos.stiff ? rs->eos.gamma * rs->eos.p_inf : 0.0;
+    int periodic[3];
+    for (int a = 0; a < 3; ++a) periodic[a] = a < nax && c->bc[2 * a] == FVG_BC_PERIODIC && c->bc[2 * a + 1] == FVG_BC_PERIODIC;
+    double* AL = (double*)malloc(sizeof(double) * NV * (size_t)nf);
+    double* AR = (double*)malloc(sizeof(double) * NV * (size_t)nf);
+    double* Fp = (double*)malloc(sizeof(double) * NV * (size_t)nf);
+    unsigned char* good = (unsigned char*)calloc((size_t)nf, 1);
+    unsigned char* pt = (unsigned char*)calloc((size_t)nf, 1);
+    unsigned char* full = (unsigned char*)calloc((size_t)nf, 1);
+    if (!AL || !AR || !Fp || !good || !pt || !full) {
+        free(AL); free(AR); free(Fp); free(good); free(pt); free(full);
+        return -2;
+    }
+    int has[6], wrap[3];
+    halo_sides(c, m, c->reconstruction == FVG_RECON_WENO5 ? 3 : 2, has, wrap);
+    const long long fb[3] = {0, m->nxf, m->nxf + m->nyf};
+    /* pass 1: good */
+    for (int a = 0; a < nax; ++a) {
+        const int e0 = dims[0] + (a == 0), e1 = dims[1] + (a == 1), e2 = dims[2] + (a == 2);
+        const int lo_a = has[2 * a] != 0, hi_a = has[2 * a + 1] != 0;
+#pragma omp parallel for schedule(static) num_threads(NT)
+        for (long long line = 0; line < (long long)e1 * e2; ++line) {
+            int cc[3];
+            cc[1] = (int)(line % e1);
+            cc[2] = (int)(line / e1);
+            for (int i = 0; i < e0; ++i) {
+                cc[0] = i;
+                const long long f = fb[a] + line * e0 + i;
+                const int ca = cc[a], n_a = dims[a];
+                if (!((ca > 0 && ca < n_a) || (ca == 0 && lo_a) || (ca == n_a && hi_a))) continue;
+                const double* ul = UL + NV * f;
+                const double* ur = UR + NV * f;
+                if (!cons_positive(g1, gp, ul) || !cons_positive(g1, gp, ur)) continue;
+                if (sb > 0.0) {
+                    const double pL = cons_pressure(g1, gp, ul), pR = cons_pressure(g1, gp, ur);
+                    if (fabs(pR - pL) > sb * smin(pL, pR)) continue;
+                }
+                good[f] = 1;
+            }
+        }
+    }
+    /* the transverse neighbour s = -1 / +1 of face f along t (-1: none) */
+#define NBR(f, cc, t, s, st)                                                                                               \
+    ((cc)[t] + (s) >= 0 && (cc)[t] + (s) < dims[t] ? (f) + (s) * (st)[t]                                               \
+                                                   : (!periodic[t] ? -1 : (f) - (long long)(s) * (dims[t] - 1) * (st)[t]))
+    /* pass 2: point values */
+    for (int a = 0; a < nax; ++a) {
+        const int e0 = dims[0] + (a == 0), e1 = dims[1] + (a == 1), e2 = dims[2] + (a == 2);
+        const long long st[3] = {1, e0, (long long)e0 * e1};
+#pragma omp parallel for schedule(static) num_threads(NT)
+        for (long long line = 0; line < (long long)e1 * e2; ++line) {
+            int cc[3];
+            cc[1] = (int)(line % e1);
+            cc[2] = (int)(line / e1);
+            for (int i = 0; i < e0; ++i) {
+                const long long f = fb[a] + line * e0 + i;
+                memcpy(AL + NV * f, UL + NV * f, sizeof(double) * NV);
+                memcpy(AR + NV * f, UR + NV * f, sizeof(double) * NV);
+                if (!good[f]) continue;
+                cc[0] = i;
+                long long nb[2][2];
+                int cnt = 0, ok = 1;
+                for (int t = 0; t < nax && ok; ++t) {
+                    if (t == a) continue;
+                    nb[cnt][0] = NBR(f, cc, t, -1, st);
+                    nb[cnt][1] = NBR(f, cc, t, 1, st);
+                    ok = nb[cnt][0] >= 0 && nb[cnt][1] >= 0 && good[nb[cnt][0]] && good[nb[cnt][1]];
+                    ++cnt;
+                }
+                if (!ok) continue;
+                double pl[NV], pr[NV];
+                for (int q = 0; q < NV; ++q) {
+                    double dl = 0.0, dr = 0.0;
+                    for (int u = 0; u < cnt; ++u) {
+                        dl += UL[NV * nb[u][1] + q] - 2.0 * UL[NV * f + q] + UL[NV * nb[u][0] + q];
+                        dr += UR[NV * nb[u][1] + q] - 2.0 * UR[NV * f + q] + UR[NV * nb[u][0] + q];
+                    }
+                    pl[q] = UL[NV * f + q] - dl / 24.0;
+                    pr[q] = UR[NV * f + q] - dr / 24.0;
+                }
+                if (!cons_positive(g1, gp, pl) || !cons_positive(g1, gp, pr)) continue;
+                memcpy(AL + NV * f, pl, sizeof pl);
+                memcpy(AR + NV * f, pr, sizeof pr);
+                pt[f] = 1;
+            }
+        }
+    }
+    /* pass 3: full */
+    for (int a = 0; a < nax; ++a) {
+        const int e0 = dims[0] + (a == 0), e1 = dims[1] + (a == 1), e2 = dims[2] + (a == 2);
+        const long long st[3] = {1, e0, (long long)e0 * e1};
+#pragma omp parallel for schedule(static) num_threads(NT)
+        for (long long line = 0; line < (long long)e1 * e2; ++line) {
+            int cc[3];
+            cc[1] = (int)(line % e1);
+            cc[2] = (int)(line / e1);
+            for (int i = 0; i < e0; ++i) {
+                const long long f = fb[a] + line * e0 + i;
+                if (!pt[f]) continue;
+                cc[0] = i;
+                int ok = 1;
+                for (int t = 0; t < nax && ok; ++t) {
+                    if (t == a) continue;
+                    const long long g0 = NBR(f, cc, t, -1, st), g1n = NBR(f, cc, t, 1, st);
+                    ok = g0 >= 0 && g1n >= 0 && pt[g0] && pt[g1n];
+                }
+                full[f] = (unsigned char)ok;
+            }
+        }
+    }
+    /* the Riemann fluxes of the point values (compute_flux, Schemes.cpp:294-318) */
+    double S = 0.0;
+    const long long nxf = m->nxf, nxyf = m->nxf + m->nyf;
+#pragma omp parallel num_threads(NT)
+    {
+        double local = 0.0;
+#pragma omp for schedule(static)
+        for (long long f = 0; f < nf; ++f) {
+            double sm;
+            face_flux(rs, AL + NV * f, AR + NV * f, f < nxf ? 0 : (f < nxyf ? 1 : 2), Fp + NV * f, &sm);
+            if (sm > local) local = sm;
+        }
+#pragma omp critical(fvo_flux4_max)
+        if (local > S) S = local;
+    }
+    /* pass 4: the corrected fluxes */
+    for (int a = 0; a < nax; ++a) {
+        const int e0 = dims[0] + (a == 0), e1 = dims[1] + (a == 1), e2 = dims[2] + (a == 2);
+        const long long st[3] = {1, e0, (long long)e0 * e1};
+#pragma omp parallel for schedule(static) num_threads(NT)
+        for (long long line = 0; line < (long long)e1 * e2; ++line) {
+            int cc[3];
+            cc[1] = (int)(line % e1);
+            cc[2] = (int)(line / e1);
+            for (int i = 0; i < e0; ++i) {
+                const long long f = fb[a] + line * e0 + i;
+                cc[0] = i;
+                if (full[f]) {
+                    long long nb[2][2];
+                    int cnt = 0;
+                    for (int t = 0; t < nax; ++t) {
+                        if (t == a) continue;
+                        nb[cnt][0] = NBR(f, cc, t, -1, st);
+                        nb[cnt][1] = NBR(f, cc, t, 1, st);
+                        ++cnt;
+                    }
+                    for (int q = 0; q < NV; ++q) {
+                        double d = 0.0;
+                        for (int u = 0; u < cnt; ++u) d += Fp[NV * nb[u][1] + q] - 2.0 * Fp[NV * f + q] + Fp[NV * nb[u][0] + q];
+                        F[NV * f + q] = Fp[NV * f + q] + d / 24.0;
+                    }
+                } else if (pt[f]) {
+                    double sm;
+                    face_flux(rs, UL + NV * f, UR + NV * f, a, F + NV * f, &sm);
+                } else {
+                    memcpy(F + NV * f, Fp + NV * f, sizeof(double) * NV);
+                }
+            }
+        }
+    }
+#undef NBR
+    free(AL); free(AR); free(Fp); free(good); free(pt); free(full);
+    *S_out = S;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -1314,6 +1578,8 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
     const Riemann rs = make_riemann(c);
     const Eos* e = &rs.eos;
     const int recon = c->reconstruction != FVG_RECON_NONE;
+    const int fourth = c->flux_quadrature == FVG_QUAD_FOURTH;
+    if (fourth && c->reconstruction != FVG_RECON_WENO3 && c->reconstruction != FVG_RECON_WENO5) return -3;
     for (int s = 0; s < 6; ++s)
         if (c->bc[s] == FVG_BC_NONREFLECTING || c->bc[s] == FVG_BC_MASSFLOW_INLET) return -3;
     double* UL = (double*)malloc(sizeof(double) * NV * (size_t)m.nf);
@@ -1330,8 +1596,11 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
         int has[6], wrap[3];
         halo_sides(c, &m, Dh, has, wrap);
         Padded P;
-        rc = pad_primitives(c, &m, e, U, Dh, has, wrap, &P);
-        if (rc == 0) reconstruct_faces(c, &m, e, &P, has, UL, UR);
+        rc = pad_primitives(c, &m, e, U, Dh, has, wrap, &P, !fourth);
+        if (rc == 0) {
+            if (fourth) reconstruct_conserved(c, &m, e, &P, has, UL, UR);
+            else reconstruct_faces(c, &m, e, &P, has, UL, UR);
+        }
         free(P.W);
         if (rc == 0 && half_dt > 0.0) {
             rev = hancock_predict(c, &m, e, half_dt, wrap, UL, UR);
@@ -1345,6 +1614,12 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
     /* RiemannSolver::compute_flux (Schemes.cpp:294-318): every face; the largest wave speed, NaN dropped */
     double S = 0.0;
     const long long nxf = m.nxf, nxyf = m.nxf + m.nyf;
+    if (fourth) {
+        if (fourth_order_fluxes(c, &m, &rs, UL, UR, F, &S) != 0) {
+            free(UL); free(UR); free(F);
+            return -2;
+        }
+    } else
 #pragma omp parallel num_threads(NT)
     {
         double local = 0.0;

@@ -116,6 +116,31 @@ def matrix():
     return out
 
 
+# flux_quadrature = fourth: WENO3-Z / WENO5-Z of the conserved averages with the transverse correction, over periodic
+# and wall boundaries (ghost slabs with boundary_order 2, first-order side faces with 1), viscous and low-Mach, the
+# shock sensor on and off, and a periodic axis shallower than the halo (no halo, yet the corrections wrap)
+def fourth_matrix():
+    F = dict(flux_quadrature="fourth")
+    walls = dict(bc_xmin="transmissive", bc_xmax="wall", bc_ymin="no_slip", bc_ymax="wall")
+    walls3 = dict(walls, bc_zmin="wall", bc_zmax="no_slip")
+    return [
+        ("2d-weno5-fourth", base_keys(12, 10, reconstruction="weno5", **F), "shock"),
+        ("2d-weno5-fourth-noblend", base_keys(12, 10, reconstruction="weno5", shock_blend="0", **F), "shock"),
+        ("2d-weno3-fourth-smooth", base_keys(12, 10, reconstruction="weno3", **F), "smooth"),
+        ("2d-walls-weno5-fourth-bo2", base_keys(12, 10, periodic=False, boundary_order=2, reconstruction="weno5", **walls, **F), "shock"),
+        ("2d-walls-weno3-fourth-bo1", base_keys(12, 10, periodic=False, boundary_order=1, reconstruction="weno3", **walls, **F), "shock"),
+        ("2d-walls-weno5-fourth-visc", base_keys(12, 10, periodic=False, boundary_order=2, reconstruction="weno5", use_viscous="true",
+                                                 mu="0.01", **dict(walls, bc_ymax="moving_wall", u_in="1.0"), **F), "smooth"),
+        ("3d-weno5-fourth-visc-lm", base_keys(8, 9, 7, reconstruction="weno5", use_viscous="true", mu="0.01",
+                                              riemann_solver="HLLC (Low-Mach Thornber)", **F), "smooth"),
+        ("3d-weno5-fourth-shock", base_keys(8, 9, 7, reconstruction="weno5", **F), "shock"),
+        ("3d-walls-weno3-fourth-bo2", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno3", **walls3, **F), "shock"),
+        ("3d-walls-weno5-fourth-bo1", base_keys(8, 9, 7, periodic=False, boundary_order=1, reconstruction="weno5", **walls3, **F), "shock"),
+        ("3d-thin-weno5-fourth-n2", base_keys(6, 4, 2, reconstruction="weno5", **F), "shock"),
+        ("2d-euler-weno5-fourth", base_keys(12, 10, reconstruction="weno5", time_scheme="Explicit Euler", **F), "smooth"),
+    ]
+
+
 def state_for(kind, keys, seed=7):
     nx, ny, nz = int(keys["nx"]), int(keys["ny"]), int(keys.get("nz", 1))
     return (shock_state if kind == "shock" else smooth_state)(nx, ny, nz, seed)
