@@ -25,6 +25,9 @@ template <int NV, int D> __global__ void k_faces_learned(KP);
 template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
 template <int NV, int D> __global__ void k_audit(KP);
 template <int NV, int D> __global__ void k_samples(KP, float*, float*);
+template <int NV, int D, int R> __global__ void k4_states(KP);
+template <int NV, int D> __global__ void k4_points(KP);
+template <int NV, int D> __global__ void k4_fluxes(KP);
 __global__ void k_indicator_control(KP, int);
 __global__ void k_totals_final(const double*, int, long long, int, double*);
 __global__ void k_control(KP, int, int);
@@ -42,9 +45,10 @@ constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along 
 constexpr int kTotalsBlocks = 1024;
 thread_local std::string g_create_error;
 
-enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_AUDIT, FAM_N };
+enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_AUDIT,
+              FAM_Q4_STATES, FAM_Q4_POINTS, FAM_Q4_FLUXES, FAM_N };
 const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_zupdate",
-                                   "k_audit"};
+                                   "k_audit", "k4_states", "k4_points", "k4_fluxes"};
 
 struct Err {
     int code;
@@ -85,6 +89,9 @@ struct fvg_ctx {
     bool graphs = true;                // fvg_advance replays a captured graph of steps
     bool fused = false;                // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
     hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
+    bool quad4 = false;                // flux_quadrature = fourth
+    double *Wc = nullptr, *q4_UL = nullptr, *q4_UR = nullptr, *q4_Fp = nullptr;
+    unsigned char *q4_good = nullptr, *q4_pt = nullptr;
 };
 
 namespace {
@@ -183,6 +190,29 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
             if constexpr (NV == 5)
                 if ((rc = LFACES(k_faces_learned, FAM_FACES_Z, 2, faces_grid(c, 2), faces_block(c, 2)))) return rc;
 #undef LFACES
+            return FVG_OK;
+        }
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        if (mode == MODE_FOURTH) {   // per axis: face states and flags, point values and their fluxes, corrected fluxes
+            const bool w5 = c->cfg.reconstruction == FVG_RECON_WENO5;
+#define Q4(KERNEL, FAM, D) \
+    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+#define Q4_STATES(D)                                                                                              \
+    launch(c, FAM_Q4_STATES, [&] {                                                                                \
+        if (w5) hipLaunchKernelGGL((k4_states<NV, D, 3>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); \
+        else hipLaunchKernelGGL((k4_states<NV, D, 2>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p);    \
+    })
+#define Q4_AXIS(D) \
+    if ((rc = Q4_STATES(D)) || (rc = Q4(k4_points, FAM_Q4_POINTS, D)) || (rc = Q4(k4_fluxes, FAM_Q4_FLUXES, D))) return rc;
+            Q4_AXIS(0)
+            Q4_AXIS(1)
+            if constexpr (NV == 5) {
+                Q4_AXIS(2)
+            }
+#undef Q4_AXIS
+#undef Q4_STATES
+#undef Q4
             return FVG_OK;
         }
     }
@@ -330,6 +360,7 @@ int fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* 
 
 int plain_mode(const fvg_ctx* c) {
     if (c->learned) return MODE_LEARNED;
+    if (c->quad4) return MODE_FOURTH;
     return c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_PLAIN;
 }
 
@@ -361,7 +392,7 @@ int enqueue_step(fvg_ctx* c) {
         return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->Ut))) return rc;
-    if (c->fused) {   // stages 2-3: the last axis' faces fused with the update; the SSP-RK3 stage-2 output goes to Rb
+    if (c->fused && mode != MODE_FOURTH) {   // stages 2-3: the last axis' faces fused with the update; the SSP-RK3 stage-2 output goes to Rb
         if (ts == FVG_TS_SSPRK2) {
             if ((rc = fused_stage(c, c->Ut, mode, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
             if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
@@ -442,8 +473,9 @@ void free_ctx(fvg_ctx* c) {
     if (c->graph) hipGraphExecDestroy(c->graph);
     hipFree(c->mlp);
     hipFree(c->ind);
-    void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,       c->HS,   c->Fc[0], c->Fc[1],
-                    c->Fc[2], c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd};
+    void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,         c->HS,  c->Fc[0], c->Fc[1], c->Fc[2],
+                    c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd, c->Wc,    c->q4_UL, c->q4_UR,
+                    c->q4_Fp, c->q4_good, c->q4_pt};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->st) hipFree(c->st);
@@ -624,6 +656,8 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
     if (cfg.flux_precision == FVG_FLUX_FP32 && cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE)
         return set_err(nullptr, FVG_EUNSUPPORTED, "flux_precision = fp32 with MUSCL-Hancock is not on the device path");
+    if (cfg.flux_quadrature == FVG_QUAD_FOURTH && cfg.flux_precision == FVG_FLUX_FP32)
+        return set_err(nullptr, FVG_EUNSUPPORTED, "flux_quadrature = fourth is on the FP64 device path only (flux_precision = fp64)");
     // face and row-block indices are 32-bit: every axis' face count stays below 2^31
     const long long fmax = std::max({(long long)(cfg.nx + 1) * cfg.ny * cfg.nz, (long long)cfg.nx * (cfg.ny + 1) * cfg.nz,
                                      (long long)cfg.nx * cfg.ny * (cfg.nz + 1)});
@@ -667,6 +701,16 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d], fe);
         if (visc && !c->fp32) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d], fe);   // fp32: combined with Fc
     }
+    c->quad4 = cfg.flux_quadrature == FVG_QUAD_FOURTH;
+    const long long q4_n = std::max({c->nfa[0], c->nfa[1], c->nfa[2]});
+    if (c->quad4) {   // the padded conservative state and one axis' face states, flags and point-value fluxes
+        rc = rc ? rc : checked_alloc(c, &c->Wc, (size_t)NV * npad);
+        rc = rc ? rc : checked_alloc(c, &c->q4_UL, (size_t)NV * q4_n);
+        rc = rc ? rc : checked_alloc(c, &c->q4_UR, (size_t)NV * q4_n);
+        rc = rc ? rc : checked_alloc(c, &c->q4_Fp, (size_t)NV * q4_n);
+        rc = rc ? rc : checked_alloc(c, &c->q4_good, (size_t)q4_n, 1);
+        rc = rc ? rc : checked_alloc(c, &c->q4_pt, (size_t)q4_n, 1);
+    }
     rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
     rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
     c->learned = cfg.indicator == FVG_INDICATOR_LEARNED;
@@ -708,6 +752,14 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     }
     p.st = c->st;
     p.spd = c->spd;
+    p.quad4 = c->quad4 ? 1 : 0;
+    p.Wc = c->Wc;
+    p.q4_UL = c->q4_UL;
+    p.q4_UR = c->q4_UR;
+    p.q4_Fp = c->q4_Fp;
+    p.q4_good = c->q4_good;
+    p.q4_pt = c->q4_pt;
+    p.q4_n = q4_n;
     p.mlp = c->mlp;
     p.ind = c->ind;
     p.ind_audit = cfg.indicator_audit;
@@ -767,7 +819,8 @@ int fvg_residual(fvg_ctx* c, double half_dt, double* R, int ncomp, double* s_max
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
     DevState s = fresh_state(c);
-    const bool hancock = half_dt > 0.0 && c->cfg.reconstruction != FVG_RECON_NONE;
+    // with fourth-order fluxes the reference's residual takes no predictor (Simulation.cpp:991-1007)
+    const bool hancock = half_dt > 0.0 && c->cfg.reconstruction != FVG_RECON_NONE && !c->quad4;
     if (hancock && c->fp32)   // the predictor kernels are FP64 only, and the primitive planes hold floats in this mode
         return set_err(c, FVG_EUNSUPPORTED, "the MUSCL-Hancock predictor is not on the fp32-flux path");
     if (hancock && !c->HS) {

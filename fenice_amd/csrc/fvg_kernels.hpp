@@ -76,6 +76,14 @@ struct KP {
     // block index and kSlotStride elements apart, so that the atomics of a pass spread over L2 channels;
     // k_control reduces and clears them
     unsigned long long* spd;
+    // flux_quadrature = fourth (FP64): Wc holds the conservative state padded like W (the stencils of the
+    // conserved reconstruction); the face states, flags and point-value fluxes of one axis at a time are q4_UL,
+    // q4_UR, q4_Fp (NV planes of q4_n faces each) and q4_good, q4_pt
+    int quad4;
+    double* Wc;
+    double *q4_UL, *q4_UR, *q4_Fp;
+    unsigned char *q4_good, *q4_pt;
+    long long q4_n;
     // learned troubled-face indicator (FP32 fluxes, WENO5-Z)
     const MlpDev* mlp;
     IndState* ind;
@@ -92,7 +100,7 @@ constexpr int PH = 3;     // halo layers of the padded primitive planes (the WEN
 constexpr int PXA = 32;   // row alignment of the padded planes in elements (128 B in FP32, 256 B in FP64)
 constexpr int PX0 = 32;   // element of a padded row that holds i = 0 (the low x halo sits just before it)
 
-enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3 };
+enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3, MODE_FOURTH = 4 };
 enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
 enum { CTL_BEGIN_STEP = 0, CTL_STAGE1_DT = 1, CTL_STAGE_S = 2, CTL_END_RK = 3, CTL_SEED = 4, CTL_HANCOCK_DT = 5, CTL_END_HANCOCK = 6,
        CTL_RESIDUAL_S = 7, CTL_AUDIT_BEGIN = 8, CTL_AUDIT_END = 9 };

@@ -318,10 +318,9 @@ __device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, i
 // 1109-1170): along an axis with a periodic halo a cell within PH of one side writes its primitives into the halo
 // beyond the other side; with a ghost slab (boundary_order = 2) it writes the primitives of its ghost state at its
 // mirror position; a side with neither has its halo layers copy the edge cell. u: the cell's state, w: its
-// primitives as stored.
-template <int NV, typename T>
-__device__ __forceinline__ void push_halo(const KP& p, int i, int j, int k, const double* u, const T* w) {
-    T* Wo = (T*)p.W;
+// primitives as stored. CONS: the planes Wo hold the conservative state (w = u, and a ghost is stored as it is).
+template <int NV, typename T, bool CONS = false>
+__device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int k, const double* u, const T* w) {
     auto put = [&](int d, int t, const T* v) {
         const long long pc = pidx_d(p, d, i, j, k, t);
 #pragma unroll
@@ -331,8 +330,12 @@ __device__ __forceinline__ void push_halo(const KP& p, int i, int j, int k, cons
         double ug[NV], wg[NV];
         T v[NV];
         ghost_state<NV>(p.rs.eos, p.bp, side, u, ug);
-        c2p<NV>(p.rs.eos, ug, wg, (double*)nullptr);
-        to_t<NV>(wg, v);
+        if constexpr (CONS) {
+            to_t<NV>(ug, v);
+        } else {
+            c2p<NV>(p.rs.eos, ug, wg, (double*)nullptr);
+            to_t<NV>(wg, v);
+        }
         put(d, t, v);
     };
 #pragma unroll
@@ -377,16 +380,23 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             float* Wo = (float*)p.W;
 #pragma unroll
             for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
-            push_halo<NV, float>(p, i, j, k, u, w);
+            push_halo<NV, float>(p, Wo, i, j, k, u, w);
             if (p.viscous) local = smax(local, cell_gradients_f32<NV>(p, c, i, j, k, w));
         } else {
             double u[NV], w[NV];
             load_cons<NV>(p, c, u);
             c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
-            double* Wo = (double*)p.W;
+            if (p.quad4) {   // fourth-order fluxes: the padded conservative state; the primitives only for the viscous term
 #pragma unroll
-            for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
-            push_halo<NV, double>(p, i, j, k, u, w);
+                for (int q = 0; q < NV; ++q) p.Wc[q * p.npad + pc] = u[q];
+                push_halo<NV, double, true>(p, p.Wc, i, j, k, u, u);
+            }
+            if (!p.quad4 || p.viscous) {
+                double* Wo = (double*)p.W;
+#pragma unroll
+                for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
+                push_halo<NV, double>(p, Wo, i, j, k, u, w);
+            }
             if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, i, j, k, w));
         }
     }

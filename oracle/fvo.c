@@ -634,6 +634,11 @@ static inline int cons_positive(double g1, double gp, const double* u) {
 }
 static inline double cons_pressure(double g1, double gp, const double* u) { return g1 * (u[3] - 0.5 * cons_m2(u) / u[0]) - gp; }
 
+/* Branch counts of the fourth-order fluxes, collected when g_q4_collect is set (fvo_fourth_stats, tests only):
+ * faces reconstructed, admissibility fallbacks, good, pt, full, pt without full, good without pt. */
+static long long g_q4_stats[7];
+static int g_q4_collect = 0;
+
 /* MusclReconstruction::reconstruct_conserved / ConservedLines::line (Schemes.cpp:1210-1349): WENO3-Z (four cells)
  * or WENO5-Z (six cells) of the conservative averages at every face with a full stencil; a face whose two states
  * are not positive, or whose density or pressure leaves the range of its stencil cells widened by the range's own
@@ -645,17 +650,19 @@ static void reconstruct_conserved(const fvg_config* c, const Mesh* m, const Eos*
     const double we = c->weno_epsilon;
     const double g1 = e->gamma - 1.0, gp = e->stiff ? e->gamma * e->p_inf : 0.0;
     const int pad = P->pad, left = five ? 2 : 1, ncell = five ? 6 : 4;
+    long long n_up = 0, n_fb = 0;
     for (int axis = 0; axis < (m->is3d ? 3 : 2); ++axis) {
         const int n_a = axis == 0 ? nx : (axis == 1 ? ny : nz);
         const size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)P->PX : (size_t)P->PX * P->PY);
         const int ni = axis == 0 ? nx + 1 : nx, nj = axis == 1 ? ny + 1 : ny, nk = axis == 2 ? nz + 1 : nz;
         const int lo = has[2 * axis] != 0, hi = has[2 * axis + 1] != 0;
-#pragma omp parallel for schedule(static) num_threads(NT)
+#pragma omp parallel for schedule(static) num_threads(NT) reduction(+ : n_up, n_fb)
         for (int kj = 0; kj < nk * nj; ++kj) {
             const int j = kj % nj, k = kj / nj;
             for (int i = 0; i < ni; ++i) {
                 const int t = axis == 0 ? i : (axis == 1 ? j : k);
                 if (!((t > 0 && t < n_a) || (t == 0 && lo) || (t == n_a && hi))) continue;
+                ++n_up;
                 const long long f = axis == 0 ? XF(m, i, j, k) : (axis == 1 ? YF(m, i, j, k) : ZF(m, i, j, k));
                 const int pi = axis == 0 ? t : i + pad, pj = axis == 1 ? t : j + pad, pk = axis == 2 ? t : k + P->zp;
                 const size_t base = ((size_t)pk * P->PY + pj) * P->PX + pi;
@@ -695,12 +702,17 @@ static void reconstruct_conserved(const fvg_config* c, const Mesh* m, const Eos*
                     }
                 }
                 if (ok) continue;
+                ++n_fb;
                 for (int q = 0; q < NV; ++q) {
                     ul[q] = P->W[q * P->plane + base + left * stride];
                     ur[q] = P->W[q * P->plane + base + (left + 1) * stride];
                 }
             }
         }
+    }
+    if (g_q4_collect) {
+        g_q4_stats[0] += n_up;
+        g_q4_stats[1] += n_fb;
     }
 }
 
@@ -881,6 +893,14 @@ static int fourth_order_fluxes(const fvg_config* c, const Mesh* m, const Riemann
         }
     }
 #undef NBR
+    if (g_q4_collect)
+        for (long long f = 0; f < nf; ++f) {
+            g_q4_stats[2] += good[f];
+            g_q4_stats[3] += pt[f];
+            g_q4_stats[4] += full[f];
+            g_q4_stats[5] += pt[f] && !full[f];
+            g_q4_stats[6] += good[f] && !pt[f];
+        }
     free(AL); free(AR); free(Fp); free(good); free(pt); free(full);
     *S_out = S;
     return 0;
@@ -1676,6 +1696,20 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
     if (reverted) *reverted = rev;
     *smax_out = S;
     return 0;
+}
+
+int fvo_fourth_stats(const fvg_config* c, const double* U, long long out[7]) {
+    const Mesh m = make_mesh(c);
+    double* R = (double*)malloc(sizeof(double) * NV * (size_t)m.nc);
+    if (!R) return -2;
+    memset(g_q4_stats, 0, sizeof g_q4_stats);
+    g_q4_collect = 1;
+    double S;
+    const int rc = fvo_residual(c, U, R, 0.0, &S, NULL);
+    g_q4_collect = 0;
+    free(R);
+    memcpy(out, g_q4_stats, sizeof g_q4_stats);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------------------------------ */

@@ -33,6 +33,9 @@ void fvo_face_flux_batch(const fvg_config* cfg, const double* UL, const double* 
  * NULL) receives the number of predicted states left as they were. Returns 0, or a negative value for a
  * configuration the oracle does not restate. */
 int fvo_residual(const fvg_config* cfg, const double* U, double* R, double half_dt, double* smax, long long* reverted);
+/* flux_quadrature = fourth: the residual's branch counts (faces reconstructed, admissibility fallbacks, good, pt,
+ * full, pt without full, good without pt), for tests that check which branches a state exercises */
+int fvo_fourth_stats(const fvg_config* cfg, const double* U, long long out[7]);
 
 /* TimeScheme::step of cfg->time_scheme (TimeScheme.cpp:110-319) on U in place. lagged_s: the
  * MUSCL-Hancock wave speed of the previous step (in/out; 0 = none yet). Returns 0, or -1 when the

@@ -65,6 +65,16 @@ class Oracle:
             raise RuntimeError(f"fvo_residual failed ({rc})")
         return R, S.value
 
+    def fourth_stats(self, cfg, U):
+        """Branch counts of the fourth-order residual of U: dict of faces reconstructed, fallbacks (admissibility),
+        good, pt, full, pt_only (pt without full), good_only (good without pt)."""
+        U = np.ascontiguousarray(U, np.float64)
+        out = (C.c_longlong * 7)()
+        rc = self.lib.fvo_fourth_stats(C.byref(cfg), _dp(U), out)
+        if rc != 0:
+            raise RuntimeError(f"fvo_fourth_stats failed ({rc})")
+        return dict(zip(("reconstructed", "fallbacks", "good", "pt", "full", "pt_only", "good_only"), list(out)))
+
     def step(self, cfg, U, t, t_max, dt_max=0.0, lagged=0.0):
         """One step in place on U; returns (dt, s_max, lagged_speed, ok)."""
         lag = C.c_double(lagged)

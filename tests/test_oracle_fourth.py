@@ -109,3 +109,16 @@ def test_fourth_needs_weno():
     with pytest.raises(InvalidArgument, match="flux_quadrature must be midpoint or fourth"):
         config_from_keys(dict(nx="8", ny="8", reconstruction="weno5", flux_quadrature="simpson"))
     assert config_from_keys(dict(nx="8", ny="8", reconstruction="weno5", flux_quadrature="Fourth")).flux_quadrature == 1
+
+
+def test_fourth_matrix_exercises_every_branch(oracle):
+    # over the configurations the device parity runs on: admissibility fallbacks, faces the shock sensor or the
+    # positivity turns away, point values without the full correction, and full corrections
+    total = dict.fromkeys(("reconstructed", "fallbacks", "good", "pt", "full", "pt_only", "good_only"), 0)
+    for _, keys, kind in FOURTH:
+        st = oracle.fourth_stats(config_from_keys(keys), state_for(kind, keys))
+        assert st["full"] + st["pt_only"] == st["pt"] and st["pt"] + st["good_only"] == st["good"] <= st["reconstructed"]
+        for k in total:
+            total[k] += st[k]
+    assert total["fallbacks"] > 0 and total["good"] < total["reconstructed"]
+    assert total["pt_only"] > 0 and total["good_only"] > 0 and total["full"] > 0
