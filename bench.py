@@ -71,7 +71,7 @@ def error_vs_fp64(n: int, flux: str, indicator: str = "none", weights: str = "",
 # CPU baseline (runs in a child process): the reference's own code built from its sources (oracle/_ref), or the
 # oracle where that build is absent, on the same workload, a bounded sample of steps.
 # ---------------------------------------------------------------------------------------------------------------
-def cpu_baseline_worker(n: int, budget_s: float) -> dict:
+def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint") -> dict:
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libfenice_ref.so")
     U0 = tgv_state(n)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
@@ -81,7 +81,7 @@ def cpu_baseline_worker(n: int, budget_s: float) -> dict:
         L.ref_create.restype = C.c_void_p
         L.ref_set_state.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ref_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        keys = run_keys(n)
+        keys = run_keys(n, "fp64", quadrature)
         ks = [k.encode() for k in keys]
         vs = [str(v).encode() for v in keys.values()]
         # a run description of our own for the reference's case loader (every key of the run is an override below)
@@ -106,7 +106,7 @@ def cpu_baseline_worker(n: int, budget_s: float) -> dict:
         from oracle_lib import Oracle
         from fenice_amd.fvg import config_from_keys
         ora = Oracle()
-        cfg = config_from_keys(run_keys(n))
+        cfg = config_from_keys(run_keys(n, "fp64", quadrature))
         U = U0.copy()
 
         def step(t):
@@ -122,15 +122,16 @@ def cpu_baseline_worker(n: int, budget_s: float) -> dict:
         t += step(t)
     el = time.perf_counter() - t0
     return {"value": n ** 3 * 3 * steps / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": kind,
-            "sample": f"3-D Taylor-Green {n}^3 WENO5-Z+HLLC+SSP-RK3 NS, {steps} timed step(s) after 1 untimed warm-up step "
+            "sample": f"3-D Taylor-Green {n}^3 WENO5-Z+HLLC+SSP-RK3 NS" + (", fourth-order flux quadrature" if quadrature == "fourth" else "") +
+                      f", {steps} timed step(s) after 1 untimed warm-up step "
                       f"({el:.1f} s timed), OpenMP {threads} threads" + (", AVX2 kernels" if kind == "reference" else "")}
 
 
-def cpu_baseline(n: int, budget_s: float):
+def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint"):
     env = dict(os.environ)
     env.pop("FENICE_NO_SIMD", None)
-    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-budget", str(budget_s)],
-                       capture_output=True, text=True, env=env)
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-budget", str(budget_s),
+                        "--quadrature", quadrature], capture_output=True, text=True, env=env)
     if r.returncode != 0:
         return {"value": None, "unit": "Mcell-updates/s per RK stage", "cores": None, "kind": None,
                 "sample": "failed: " + (r.stderr or r.stdout)[-300:]}
@@ -271,6 +272,40 @@ def side_workload(name: str) -> dict:
             m["vs_fp64"] = round(m["value"] / base, 3)
         return {"workload": "3-D Taylor-Green vortex 512^3, Navier-Stokes Re 1600 Pr 0.71 Mach 0.1, WENO5-Z + HLLC + SSP-RK3, "
                             "CFL 0.5, periodic", "unit": "Mcell-updates/s per RK stage", "modes": modes}
+    if name == "fourth":
+        # the 256^3 Taylor-Green vortex with the fourth-order flux quadrature beside the midpoint one (FP64 both): 2
+        # warm-up steps, 10 timed steps with per-kernel event timing; the reference's own code on the host at 96^3
+        n = 256
+        U0 = tgv_state(n)
+        cells = n ** 3
+        nf = 3 * (n + 1) * n * n
+        modes = {}
+        for quad in ("midpoint", "fourth"):
+            with fvg.Solver(fvg.config_from_keys(run_keys(n, "fp64", quad))) as s:
+                s.upload(U0)
+                s.advance(0.0, 1e9, max_steps=2)
+                s.set_timing(True)
+                t0 = time.perf_counter()
+                _, k, _ = s.advance(0.0, 1e9, max_steps=10)
+                el = time.perf_counter() - t0
+                times = s.kernel_times()
+            m = {"value": round(cells * 3 * k / el / 1e6, 1), "ms_per_step": round(el / k * 1e3, 3), "steps": k,
+                 "kernels": {kn: {"avg_ms": round(v[0] / max(1, v[1]), 4), "launches": v[1]} for kn, v in times.items()}}
+            if quad == "fourth":
+                # algorithmic HBM bytes per face of each pass (FP64, NV = 5): states write two states and a flag
+                # (81 B) and read the padded conservative stencil once per cell (40 B); points read the states and
+                # the flag (81 B) and write the flux and a flag (41 B); fluxes read the flux and flag (41 B) and write
+                # the convective (40 B) and viscous (32 B) fluxes (the viscous inputs, W and G, are not counted)
+                per_face = {"k4_states": 121.0, "k4_points": 122.0, "k4_fluxes": 113.0}
+                for kn, b in per_face.items():
+                    if kn in times and times[kn][1]:
+                        avg_s = times[kn][0] / times[kn][1] * 1e-3
+                        m["kernels"][kn]["algorithmic_GBs"] = round(b * nf / 3 / avg_s / 1e9, 1)
+            modes[quad] = m
+        modes["fourth"]["vs_midpoint"] = round(modes["fourth"]["value"] / modes["midpoint"]["value"], 3)
+        return {"workload": "3-D Taylor-Green vortex 256^3, Navier-Stokes Re 1600 Pr 0.71 Mach 0.1, WENO5-Z + HLLC + SSP-RK3, "
+                            "CFL 0.5, periodic, FP64", "unit": "Mcell-updates/s per RK stage", "modes": modes,
+                "cpu_baseline_fourth": cpu_baseline(96, 15.0, "fourth")}
     raise ValueError(name)
 
 
@@ -289,11 +324,12 @@ def main():
     ap.add_argument("--indicator", default="none", choices=["none", "learned"], help="troubled-face indicator (learned: fp32)")
     ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
-    ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512"],
+    ap.add_argument("--quadrature", default="midpoint", choices=["midpoint", "fourth"], help="flux quadrature of the CPU worker")
+    ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512", "fourth"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
     if a.cpu_worker:
-        print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget)))
+        print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature)))
         return
     if a.side:
         print(json.dumps(side_workload(a.side)), flush=True)

@@ -123,13 +123,17 @@ def taylor_green(n: int, gamma: float = 1.4, mach: float = 0.1) -> np.ndarray:
     return U
 
 
-def taylor_green_keys(n: int, flux: str = "fp64") -> dict:
-    """Navier-Stokes at Re 1600 (mu 6.25e-4), Pr 0.71, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic."""
+def taylor_green_keys(n: int, flux: str = "fp64", quadrature: str = "midpoint") -> dict:
+    """Navier-Stokes at Re 1600 (mu 6.25e-4), Pr 0.71, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic; `quadrature`
+    "fourth" takes the fourth-order flux quadrature (FP64)."""
     L = "6.283185307179586"
-    return dict(nx=n, ny=n, nz=n, Lx=L, Ly=L, Lz=L, gamma=1.4, use_viscous="true", mu="6.25e-4", pr="0.71",
-                bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", bc_zmin="periodic",
-                bc_zmax="periodic", riemann_solver="HLLC", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5",
-                flux_precision=flux)
+    k = dict(nx=n, ny=n, nz=n, Lx=L, Ly=L, Lz=L, gamma=1.4, use_viscous="true", mu="6.25e-4", pr="0.71",
+             bc_xmin="periodic", bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", bc_zmin="periodic",
+             bc_zmax="periodic", riemann_solver="HLLC", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5",
+             flux_precision=flux)
+    if quadrature != "midpoint":
+        k["flux_quadrature"] = quadrature
+    return k
 
 
 def vortex_state(n: int, L: float = 10.0, gamma: float = 1.4) -> np.ndarray:
