@@ -40,6 +40,7 @@ class Config(C.Structure):
         ("indicator", C.c_int), ("indicator_audit", C.c_int), ("indicator_label", C.c_double),
         ("indicator_fn_limit", C.c_double), ("indicator_fn_floor", C.c_double),
         ("flux_quadrature", C.c_int),
+        ("nrbc_sigma", C.c_double), ("nrbc_beta", C.c_double),
     ]
 
     @staticmethod
@@ -62,6 +63,7 @@ class Config(C.Structure):
         c.indicator, c.indicator_audit, c.indicator_label, c.indicator_fn_limit = INDICATOR_NONE, 50, 0.05, 0.01
         c.indicator_fn_floor = 1e-4
         c.flux_quadrature = QUAD_MIDPOINT
+        c.nrbc_sigma, c.nrbc_beta = 0.28, -1.0
         return c
 
     @staticmethod

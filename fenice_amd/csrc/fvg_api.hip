@@ -12,6 +12,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <cstddef>
+
 #include "../../include/fvg.h"
 #include "fvg_host.hpp"
 
@@ -31,6 +33,8 @@ template <int NV, int D> __global__ void k4_fluxes(KP);
 __global__ void k_indicator_control(KP, int);
 __global__ void k_totals_final(const double*, int, long long, int, double*);
 __global__ void k_control(KP, int, int);
+__global__ void k_massflow(KP);
+template <int NV, int SIDE> __global__ void k_nrbc(KP);
 __global__ void k_riemann_batch(RiemannParams, const double*, const double*, const int*, long long, double*, double*);
 }   // namespace fvg
 
@@ -90,6 +94,9 @@ struct fvg_ctx {
     bool fused = false;                // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
     hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
     bool quad4 = false;                // flux_quadrature = fourth
+    bool massflow = false;             // a mass-flow inlet side: k_massflow runs before every k_cells
+    int nrbc_sides = 0;                // bit s: side s is non-reflecting (k_nrbc runs for it before every k_cells)
+    double* nrbc = nullptr;            // KP::nrbc
     double *Wc = nullptr, *q4_UL = nullptr, *q4_UR = nullptr, *q4_Fp = nullptr;
     unsigned char *q4_good = nullptr, *q4_pt = nullptr;
 };
@@ -169,10 +176,44 @@ KP make_kp(fvg_ctx* c, const double* Uin) {
     return p;
 }
 
+// the mass-flow targets and the non-reflecting ghosts of the stage input, which the ghost states of k_cells and the
+// face passes read
+int enqueue_massflow(fvg_ctx* c, const KP& p) {
+    int rc = FVG_OK;
+    if (c->massflow && (rc = launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_massflow, dim3(6), dim3(kBlock), 0, c->stream, p); })))
+        return rc;
+    for (int s = 0; s < 6 && !rc; ++s) {
+        if (!(c->nrbc_sides >> s & 1)) continue;
+        const long long n = s < 2 ? (long long)c->cfg.ny * c->cfg.nz : (s < 4 ? (long long)c->cfg.nx * c->cfg.nz : (long long)c->cfg.nx * c->cfg.ny);
+        const dim3 g(grid_for(n)), b(kBlock);
+        rc = launch(c, FAM_CONTROL, [&] {
+            if (c->NV == 4) {
+                switch (s) {
+                case 0: hipLaunchKernelGGL((k_nrbc<4, 0>), g, b, 0, c->stream, p); break;
+                case 1: hipLaunchKernelGGL((k_nrbc<4, 1>), g, b, 0, c->stream, p); break;
+                case 2: hipLaunchKernelGGL((k_nrbc<4, 2>), g, b, 0, c->stream, p); break;
+                default: hipLaunchKernelGGL((k_nrbc<4, 3>), g, b, 0, c->stream, p); break;
+                }
+            } else {
+                switch (s) {
+                case 0: hipLaunchKernelGGL((k_nrbc<5, 0>), g, b, 0, c->stream, p); break;
+                case 1: hipLaunchKernelGGL((k_nrbc<5, 1>), g, b, 0, c->stream, p); break;
+                case 2: hipLaunchKernelGGL((k_nrbc<5, 2>), g, b, 0, c->stream, p); break;
+                case 3: hipLaunchKernelGGL((k_nrbc<5, 3>), g, b, 0, c->stream, p); break;
+                case 4: hipLaunchKernelGGL((k_nrbc<5, 4>), g, b, 0, c->stream, p); break;
+                default: hipLaunchKernelGGL((k_nrbc<5, 5>), g, b, 0, c->stream, p); break;
+                }
+            }
+        });
+    }
+    return rc;
+}
+
 template <int NV, typename T>
 int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
     const KP p = make_kp(c, Uin);
     int rc;
+    if ((rc = enqueue_massflow(c, p))) return rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     if constexpr (std::is_same<T, float>::value) {
@@ -250,6 +291,7 @@ template <int NV>
 int enqueue_hancock_residual(fvg_ctx* c, const double* Uin) {
     const KP p = make_kp(c, Uin);
     int rc;
+    if ((rc = enqueue_massflow(c, p))) return rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, double>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
@@ -300,6 +342,7 @@ template <int NV, typename T>
 int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
     const KP p = make_kp(c, Uin);
     int rc;
+    if ((rc = enqueue_massflow(c, p))) return rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
     for (int d = 0; d < (NV == 5 ? 2 : 1); ++d) {
@@ -475,7 +518,7 @@ void free_ctx(fvg_ctx* c) {
     hipFree(c->ind);
     void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,         c->HS,  c->Fc[0], c->Fc[1], c->Fc[2],
                     c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd, c->Wc,    c->q4_UL, c->q4_UR,
-                    c->q4_Fp, c->q4_good, c->q4_pt};
+                    c->q4_Fp, c->q4_good, c->q4_pt, c->nrbc};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->st) hipFree(c->st);
@@ -586,6 +629,13 @@ void fill_kernel_params(const fvg_config& cfg, KP& p) {
     p.bp.u_in = cfg.u_in;
     p.bp.v_in = cfg.v_in;
     p.bp.p_out = cfg.p_out;
+    p.bp.rho_in = cfg.rho_in;
+    p.bp.nrbc_sigma = cfg.nrbc_sigma;
+    p.bp.nrbc_beta = cfg.nrbc_beta;
+    p.bp.length[0] = cfg.Lx;
+    p.bp.length[1] = cfg.Ly;
+    p.bp.length[2] = cfg.Lz;
+    p.bp.ut = nullptr;   // the device state's ut, set by fvg_create
     p.recon = cfg.reconstruction;
     p.lim = cfg.limiter;
     p.Dh = cfg.reconstruction == FVG_RECON_WENO5 ? 3 : 2;
@@ -646,12 +696,6 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     fvg_config cfg = *cfg_in;
     char msg[256] = {0};
     if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return set_err(nullptr, FVG_EINVAL, msg);
-    for (int s = 0; s < 6; ++s) {
-        const int b = cfg.bc[s];
-        if ((s < 4 || cfg.nz > 1) && (b == FVG_BC_NONREFLECTING || b == FVG_BC_MASSFLOW_INLET))
-            return set_err(nullptr, FVG_EUNSUPPORTED, "boundary type of side " + std::to_string(s) +
-                                                          " (non-reflecting / mass-flow inlet) is not on the device path");
-    }
     if (cfg.time_scheme == FVG_TS_HANCOCK && (cfg.reconstruction == FVG_RECON_WENO3 || cfg.reconstruction == FVG_RECON_WENO5))
         return set_err(nullptr, FVG_EUNSUPPORTED, "MUSCL-Hancock with a WENO reconstruction is not on the device path");
     if (cfg.flux_precision == FVG_FLUX_FP32 && cfg.time_scheme == FVG_TS_HANCOCK && cfg.reconstruction != FVG_RECON_NONE)
@@ -711,6 +755,14 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         rc = rc ? rc : checked_alloc(c, &c->q4_good, (size_t)q4_n, 1);
         rc = rc ? rc : checked_alloc(c, &c->q4_pt, (size_t)q4_n, 1);
     }
+    long long nrbc_off[6] = {0, 0, 0, 0, 0, 0}, nrbc_total = 0;
+    for (int s = 0; s < (three ? 6 : 4); ++s) {
+        if (cfg.bc[s] != FVG_BC_NONREFLECTING) continue;
+        c->nrbc_sides |= 1 << s;
+        nrbc_off[s] = nrbc_total;
+        nrbc_total += (long long)NV * (s < 2 ? (long long)cfg.ny * cfg.nz : (s < 4 ? (long long)cfg.nx * cfg.nz : (long long)cfg.nx * cfg.ny));
+    }
+    rc = rc ? rc : checked_alloc(c, &c->nrbc, (size_t)nrbc_total);
     rc = rc ? rc : checked_alloc(c, &c->tot_part, (size_t)3 * kTotalsBlocks);
     rc = rc ? rc : checked_alloc(c, &c->tot_out, 3);
     c->learned = cfg.indicator == FVG_INDICATOR_LEARNED;
@@ -752,6 +804,10 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     }
     p.st = c->st;
     p.spd = c->spd;
+    p.bp.ut = reinterpret_cast<const double*>(reinterpret_cast<const char*>(c->st) + offsetof(DevState, ut));
+    p.nrbc = c->nrbc;
+    for (int s = 0; s < 6; ++s) p.nrbc_off[s] = nrbc_off[s];
+    for (int s = 0; s < (three ? 6 : 4); ++s) c->massflow = c->massflow || cfg.bc[s] == FVG_BC_MASSFLOW_INLET;
     p.quad4 = c->quad4 ? 1 : 0;
     p.Wc = c->Wc;
     p.q4_UL = c->q4_UL;
@@ -1031,6 +1087,11 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
     KP p = make_kp(c, c->U);
     p.viscous = 0;   // only the primitives are needed: no gradients, and no speed maxima left behind in the slots
     const dim3 gc(cells_grid(c)), bc(cells_block(c)), gf((unsigned)((nfaces + kBlock - 1) / kBlock)), b(kBlock);
+    if (int rc = enqueue_massflow(c, p)) {
+        hipFree(dX);
+        hipFree(dd);
+        return rc;
+    }
     if (c->NV == 5) {
         hipLaunchKernelGGL((k_cells<5, float>), gc, bc, 0, c->stream, p);
         if (axis == 0) hipLaunchKernelGGL((k_samples<5, 0>), gf, b, 0, c->stream, p, dX, dd);

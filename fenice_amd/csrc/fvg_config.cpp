@@ -88,6 +88,8 @@ void fvg_config_defaults(fvg_config* c) {
     c->indicator_fn_limit = 0.01;
     c->indicator_fn_floor = 1e-4;
     c->flux_quadrature = FVG_QUAD_MIDPOINT;
+    c->nrbc_sigma = 0.28;
+    c->nrbc_beta = -1.0;
 }
 
 int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* err, size_t n) {
@@ -179,6 +181,8 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
     if (key == "indicator_label") return num(c->indicator_label);
     if (key == "indicator_fn_limit") return num(c->indicator_fn_limit);
     if (key == "indicator_fn_floor") return num(c->indicator_fn_floor);
+    if (key == "nrbc_sigma") return num(c->nrbc_sigma);   // Params.cpp:274-275
+    if (key == "nrbc_beta") return num(c->nrbc_beta);
     if (key == "flux_quadrature") {   // Params.cpp:348, 448-449
         if (lv == "midpoint") return (c->flux_quadrature = FVG_QUAD_MIDPOINT), FVG_OK;
         if (lv == "fourth") return (c->flux_quadrature = FVG_QUAD_FOURTH), FVG_OK;
@@ -227,6 +231,8 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
     if (c->flux_quadrature == FVG_QUAD_FOURTH && c->reconstruction != FVG_RECON_WENO3 && c->reconstruction != FVG_RECON_WENO5)
         return fail(err, n, "flux_quadrature = fourth needs reconstruction = weno3 or weno5");
     if (c->boundary_order != 1 && c->boundary_order != 2) return fail(err, n, "boundary_order must be 1 or 2");
+    if (!(c->nrbc_sigma >= 0.0)) return fail(err, n, "nrbc_sigma must be >= 0");   // Params.cpp:489-490
+    if (!(c->nrbc_beta <= 1.0)) return fail(err, n, "nrbc_beta must be <= 1 (< 0: local Mach number)");
     if (c->indicator == FVG_INDICATOR_LEARNED &&
         (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))
         return fail(err, n, "indicator = learned needs reconstruction = weno5, flux_precision = fp32 and a Runge-Kutta scheme");

@@ -287,6 +287,10 @@ __device__ __forceinline__ T weno5z(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
 struct BcParams {
     int bc[6];
     double u_in, v_in, p_out;
+    double rho_in;                   // mass-flow inlet density
+    double nrbc_sigma, nrbc_beta;    // non-reflecting outlet: relaxation towards p_out, transverse weight
+    double length[3];                // domain size per axis (the non-reflecting relaxation length)
+    const double* ut;                // mass-flow inlet velocity per side, set on the device each residual (or null)
 };
 
 // The boundary type of a side, without indexing the parameter array by a run-time value (that would put the
@@ -302,7 +306,8 @@ __device__ __forceinline__ int bc_of(const BcParams& b, int side) {
     }
 }
 
-// side: 0 left, 1 right, 2 bottom, 3 top, 4 front, 5 back. Periodic, non-reflecting and mass-flow sides never reach here.
+// side: 0 left, 1 right, 2 bottom, 3 top, 4 front, 5 back. Periodic and non-reflecting sides never reach here; a
+// mass-flow side reads its target velocity from b.ut (set by k_massflow before the residual's first pass).
 template <int NV>
 __device__ __forceinline__ void ghost_state(const Eos& e, const BcParams& b, int side, const double* Ui, double* Ug) {
     const int bc = bc_of(b, side);
@@ -329,6 +334,10 @@ __device__ __forceinline__ void ghost_state(const Eos& e, const BcParams& b, int
         if constexpr (NV == 5) Wg[4] = -Wi[4];
     } else if (bc == 2 || bc == 3 /* moving wall, velocity inlet */) {
         Wg[1] = 2.0 * b.u_in - Wi[1];
+        Wg[2] = 2.0 * b.v_in - Wi[2];
+        if constexpr (NV == 5) Wg[4] = -Wi[4];
+    } else if (bc == 4 /* mass-flow inlet: the side's target velocity */) {
+        Wg[1] = 2.0 * b.ut[side] - Wi[1];
         Wg[2] = 2.0 * b.v_in - Wi[2];
         if constexpr (NV == 5) Wg[4] = -Wi[4];
     } else if (bc == 5 /* outlet */) {

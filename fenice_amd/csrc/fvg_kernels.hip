@@ -31,6 +31,55 @@ __device__ __forceinline__ void end_step(DevState* s) {
     }
 }
 
+template __global__ void k_nrbc<4, 0>(KP);
+template __global__ void k_nrbc<4, 1>(KP);
+template __global__ void k_nrbc<4, 2>(KP);
+template __global__ void k_nrbc<4, 3>(KP);
+template __global__ void k_nrbc<5, 0>(KP);
+template __global__ void k_nrbc<5, 1>(KP);
+template __global__ void k_nrbc<5, 2>(KP);
+template __global__ void k_nrbc<5, 3>(KP);
+template __global__ void k_nrbc<5, 4>(KP);
+template __global__ void k_nrbc<5, 5>(KP);
+
+// The mass-flow inlet velocity of every mass-flow side (apply_side, Boundaries.cpp:285-294): rho_in u_in over the
+// mean density of the side's boundary cells, floored at 1e-5. One block per side; the densities are summed in the
+// reference's boundary order (Core.hpp:121-151) by one thread, a chunk of the block's loads at a time, so that the
+// sum has the reference's bits. The result goes to DevState::ut, which the ghost states of the residual read.
+__global__ __launch_bounds__(256) void k_massflow(KP p) {
+    __shared__ double rho[256];
+    const int side = blockIdx.x;
+    if (bc_of(p.bp, side) != 4 || (side >= 4 && !p.is3d) || p.st->done) return;   // uniform over the block
+    const long long nx = p.nx, ny = p.ny, nz = p.nz;
+    const long long n = side < 2 ? ny * nz : (side < 4 ? nx * nz : nx * ny);
+    double sum = 0.0;
+    for (long long base = 0; base < n; base += blockDim.x) {
+        const long long k = base + threadIdx.x;
+        if (k < n) {
+            long long c;
+            switch (side) {
+            case 0: c = k * nx; break;
+            case 1: c = k * nx + nx - 1; break;
+            case 2: c = (k / nx) * ny * nx + k % nx; break;
+            case 3: c = ((k / nx) * ny + ny - 1) * nx + k % nx; break;
+            case 4: c = k; break;
+            default: c = (nz - 1) * ny * nx + k; break;
+            }
+            rho[threadIdx.x] = smax(p.U[c], kEosEps);   // the density of cons_to_prim
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const long long m = n - base < (long long)blockDim.x ? n - base : (long long)blockDim.x;
+            for (long long t = 0; t < m; ++t) sum += rho[t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rho_avg = smax(sum / (double)n, 1e-5);
+        p.st->ut[side] = (p.bp.rho_in * p.bp.u_in) / rho_avg;
+    }
+}
+
 // op: the CTL_* step; arg: the time scheme (CTL_STAGE1_DT, CTL_END_RK) or the stage number (CTL_STAGE_S).
 // Launched with kSpeedSlots threads: each takes one slot of each speed maximum, clears it, and the wavefront's
 // maxima become s_conv / s_visc; the rest runs on one thread.

@@ -22,6 +22,7 @@ struct DevState {
     double s1, s2, s3, s_step, lagged;
     long long reverted;
     int done, diverged, steps, max_steps;
+    double ut[6];   // mass-flow inlet velocity per side of the current residual (k_massflow)
 };
 
 // n / d for 0 <= n < 2^31 as (umulhi(n, m) + n) >> l, with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1
@@ -84,6 +85,10 @@ struct KP {
     double *q4_UL, *q4_UR, *q4_Fp;
     unsigned char *q4_good, *q4_pt;
     long long q4_n;
+    // non-reflecting sides: the ghost state of each boundary cell of the stage input (k_nrbc), NV planes per side
+    // of the side's boundary cells, side s from element nrbc_off[s] (null without such a side)
+    double* nrbc;
+    long long nrbc_off[6];
     // learned troubled-face indicator (FP32 fluxes, WENO5-Z)
     const MlpDev* mlp;
     IndState* ind;

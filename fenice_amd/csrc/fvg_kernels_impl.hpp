@@ -131,12 +131,108 @@ __device__ __forceinline__ void stencil_prim(const KP& p, int d, int i, int j, i
     load_prim<NV, T>(p, pidx_d(p, d, i, j, k, t), w);
 }
 
+// The non-reflecting outlet's ghost of boundary cell (i, j, k) on side SIDE (nonreflecting_ghost and the tangential
+// neighbours of apply_side, Boundaries.cpp:155-198, 314-360): a characteristic outlet whose ghost differs from the
+// cell only in the incoming acoustic wave, relaxed towards p_out with the transverse terms weighted by 1 - beta.
+// Tangential derivatives are central differences of the primitives, wrapped across a periodic axis, one-sided at
+// the ends of the others. k_nrbc computes it once per stage into the side's plane of KP::nrbc, which the face
+// kernels read.
+template <int NV, int SIDE>
+__device__ __forceinline__ void nrbc_ghost(const KP& p, int i, int j, int k, double* ug) {
+    constexpr int axis = SIDE >> 1;
+    constexpr double s = (SIDE & 1) ? 1.0 : -1.0;
+    constexpr int qv = mom(axis);
+    const BcParams& b = p.bp;
+    double Wi[NV], Wg[NV];
+    load_cons<NV>(p, cidx(p, i, j, k), ug);
+    c2p<NV>(p.rs.eos, ug, Wi, (double*)nullptr);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Wg[q] = Wi[q];
+    const double u_out = s * Wi[qv];
+    const double c = sound_speed(p.rs.eos, Wi[3], Wi[0]);
+    const double M = smin(smax(u_out / c, -1.0), 1.0);
+    if (M >= 1.0) {   // supersonic outflow: nothing enters
+        p2c<NV>(p.rs.eos, Wg, ug);
+        return;
+    }
+    const double h = hstep(p, axis);
+    const double len = axis == 0 ? b.length[0] : (axis == 1 ? b.length[1] : b.length[2]);
+    const double K = b.nrbc_sigma * c * (1.0 - M * M) / len;
+    const double beta = b.nrbc_beta >= 0.0 ? b.nrbc_beta : fabs(M);
+    double T1 = 0.0;
+    if (beta < 1.0) {
+#pragma unroll
+        for (int t = 0; t < NV - 2; ++t) {
+            const int n = ncell(p, t);
+            if (t == axis || n == 1) continue;
+            const int ct = coord(t, i, j, k);
+            int lo = ct - 1, hi = ct + 1;
+            double span = 2.0;
+            if (vper_of(p, t)) {
+                lo = (lo + n) % n;
+                hi = hi % n;
+            } else {
+                if (lo < 0) {
+                    lo = 0;
+                    span -= 1.0;
+                }
+                if (hi >= n) {
+                    hi = n - 1;
+                    span -= 1.0;
+                }
+                if (span <= 0.0) continue;
+            }
+            double u[NV], Wl[NV], Wh[NV];
+            load_cons<NV>(p, cidx_d(p, t, i, j, k, lo), u);
+            c2p<NV>(p.rs.eos, u, Wl, (double*)nullptr);
+            load_cons<NV>(p, cidx_d(p, t, i, j, k, hi), u);
+            c2p<NV>(p.rs.eos, u, Wh, (double*)nullptr);
+            const double ht = span * hstep(p, t);
+            const int qt = mom(t);
+            T1 += Wi[qt] * (Wh[3] - Wl[3]) / ht + Wi[0] * c * c * (Wh[qt] - Wl[qt]) / ht -
+                  Wi[0] * c * s * Wi[qt] * (Wh[qv] - Wl[qv]) / ht;
+        }
+    }
+    const double dw1 = -h * (K * (Wi[3] - b.p_out) - (1.0 - beta) * T1) / (c - u_out);
+    Wg[3] = Wi[3] + 0.5 * dw1;
+    Wg[qv] = Wi[qv] - s * dw1 / (2.0 * Wi[0] * c);
+    Wg[0] = Wi[0] + 0.5 * dw1 / (c * c);
+    p2c<NV>(p.rs.eos, Wg, ug);
+}
+
+// boundary cells of a side (Core.hpp:121-151) and the index of cell (i, j, k) among them
+__device__ __forceinline__ long long side_cells(const KP& p, int side) {
+    return side < 2 ? (long long)p.ny * p.nz : (side < 4 ? (long long)p.nx * p.nz : (long long)p.nx * p.ny);
+}
+__device__ __forceinline__ long long side_index(const KP& p, int side, int i, int j, int k) {
+    return side < 2 ? (long long)k * p.ny + j : (side < 4 ? (long long)k * p.nx + i : (long long)j * p.nx + i);
+}
+__device__ __forceinline__ long long nrbc_off_of(const KP& p, int s) {
+    switch (s) {
+    case 0: return p.nrbc_off[0];
+    case 1: return p.nrbc_off[1];
+    case 2: return p.nrbc_off[2];
+    case 3: return p.nrbc_off[3];
+    case 4: return p.nrbc_off[4];
+    default: return p.nrbc_off[5];
+    }
+}
+
 // First-order state outside side `side` (along axis d) of the boundary cell (i, j, k) (apply_side,
-// Boundaries.cpp:259-371): the opposite cell on a periodic side, else the ghost state of the cell.
+// Boundaries.cpp:259-371): the opposite cell on a periodic side, the non-reflecting ghost, else the ghost state of
+// the cell.
 template <int NV>
 __device__ __forceinline__ void outside_first_order(const KP& p, int d, int side, int i, int j, int k, double* ug) {
-    if (bc_of(p.bp, side) == 7) {
+    const int bc = bc_of(p.bp, side);
+    if (bc == 7) {
         load_cons<NV>(p, cidx_d(p, d, i, j, k, (side & 1) ? 0 : ncell(p, d) - 1), ug);
+        return;
+    }
+    if (bc == 8) {   // the non-reflecting ghost k_nrbc stored for this boundary cell
+        const long long n = side_cells(p, side), kb = side_index(p, side, i, j, k);
+        const double* g = p.nrbc + nrbc_off_of(p, side);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) ug[q] = g[q * n + kb];
         return;
     }
     double ui[NV];
@@ -401,6 +497,33 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
         }
     }
     if (p.viscous) block_max_slot(p, local, SPD_VISC);
+}
+
+// k_nrbc: the non-reflecting ghosts of side SIDE's boundary cells (thread per cell, in the side's boundary order),
+// from the stage input, before the passes that read them
+template <int NV, int SIDE>
+__global__ __launch_bounds__(256) void k_nrbc(KP p) {
+    const long long n = side_cells(p, SIDE), kb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kb >= n || p.st->done) return;
+    int i, j, k;
+    if (SIDE < 2) {
+        i = SIDE == 0 ? 0 : p.nx - 1;
+        j = (int)(kb % p.ny);
+        k = (int)(kb / p.ny);
+    } else if (SIDE < 4) {
+        i = (int)(kb % p.nx);
+        j = SIDE == 2 ? 0 : p.ny - 1;
+        k = (int)(kb / p.nx);
+    } else {
+        i = (int)(kb % p.nx);
+        j = (int)(kb / p.nx);
+        k = SIDE == 4 ? 0 : p.nz - 1;
+    }
+    double g[NV];
+    nrbc_ghost<NV, SIDE>(p, i, j, k, g);
+    double* out = p.nrbc + p.nrbc_off[SIDE];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q * n + kb] = g[q];
 }
 
 // ------------------------------------------------------------------------------------------------------------

@@ -100,6 +100,8 @@ typedef struct fvg_config {
     double indicator_fn_limit;   /* fallback limit of the missed fraction of troubled faces (default 0.01) */
     double indicator_fn_floor;   /* fallback limit of the missed fraction of all audited faces (default 1e-4) */
     int flux_quadrature;         /* enum fvg_flux_quadrature (default MIDPOINT) */
+    double nrbc_sigma;           /* non-reflecting outlet: relaxation of p towards p_out (default 0.28; 0: none) */
+    double nrbc_beta;            /* non-reflecting outlet: transverse weight, <= 1 (default -1: the local Mach number) */
 } fvg_config;
 
 /* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,

@@ -317,14 +317,15 @@ static inline double weno5z(double c0, double c1, double c2, double c3, double c
 static int side_axis(int side) { return side / 2; }
 static int side_high(int side) { return side == FVG_RIGHT || side == FVG_TOP || side == FVG_BACK; }
 
-/* ghost_state (Boundaries.cpp:106-153); massflow inlet and non-reflecting are not restated (-1). */
-static int ghost_state(const fvg_config* c, const Eos* e, int side, const double* Ui, double* Ug) {
+/* ghost_state (Boundaries.cpp:106-153); u_target: the mass-flow inlet's velocity of this side (apply_side). A
+ * periodic or non-reflecting side has no single-cell ghost (-1). */
+static int ghost_state(const fvg_config* c, const Eos* e, int side, const double* Ui, double* Ug, double u_target) {
     const int bc = c->bc[side];
     if (bc == FVG_BC_TRANSMISSIVE) {
         for (int q = 0; q < NV; ++q) Ug[q] = Ui[q];
         return 0;
     }
-    if (bc == FVG_BC_PERIODIC || bc == FVG_BC_NONREFLECTING || bc == FVG_BC_MASSFLOW_INLET) return -1;
+    if (bc == FVG_BC_PERIODIC || bc == FVG_BC_NONREFLECTING) return -1;
     const int axis = side_axis(side);
     const double nxn = axis == 0 ? 1.0 : 0.0, nyn = axis == 1 ? 1.0 : 0.0, nzn = axis == 2 ? 1.0 : 0.0;
     double Wi[NV], Wg[NV];
@@ -346,6 +347,11 @@ static int ghost_state(const fvg_config* c, const Eos* e, int side, const double
     case FVG_BC_MOVING_WALL:
     case FVG_BC_VEL_INLET:
         Wg[1] = 2.0 * c->u_in - Wi[1];
+        Wg[2] = 2.0 * c->v_in - Wi[2];
+        Wg[4] = -Wi[4];
+        break;
+    case FVG_BC_MASSFLOW_INLET:
+        Wg[1] = 2.0 * u_target - Wi[1];
         Wg[2] = 2.0 * c->v_in - Wi[2];
         Wg[4] = -Wi[4];
         break;
@@ -397,9 +403,104 @@ static long long boundary_cell(const Mesh* m, int side, long long k) {
     }
 }
 
-/* apply_boundary_conditions + apply_side (Boundaries.cpp:54-104, 259-371), periodic / ghost_state sides. */
+/* The mass-flow inlet velocity of every side (apply_side, Boundaries.cpp:285-294): rho_in u_in over the mean
+ * density of the side's boundary cells (summed in boundary order), floored at 1e-5; 0 on the other sides. */
+static void massflow_targets(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, double ut[6]) {
+    for (int side = 0; side < 6; ++side) {
+        ut[side] = 0.0;
+        if (c->bc[side] != FVG_BC_MASSFLOW_INLET || (side >= 4 && !m->is3d)) continue;
+        const long long n = boundary_count(m, side);
+        double sum = 0.0;
+        const double count = (double)n;
+        for (long long k = 0; k < n; ++k) {
+            double W[NV];
+            c2p(e, U + NV * boundary_cell(m, side, k), W, NULL);
+            sum += W[0];
+        }
+        const double rho_avg = smax(sum / count, 1e-5);
+        ut[side] = (c->rho_in * c->u_in) / rho_avg;
+    }
+}
+
+/* nonreflecting_ghost (Boundaries.cpp:155-198) with the tangential neighbours of apply_side (Boundaries.cpp:314-360):
+ * a characteristic outlet whose ghost differs from the cell only in the incoming acoustic wave, relaxed towards
+ * p_out (nrbc_sigma, the domain length along the axis) with the transverse terms weighted by 1 - beta
+ * (beta = nrbc_beta, or |M| when nrbc_beta < 0). Tangential derivatives are central differences of the primitives,
+ * wrapped across a periodic axis, one-sided at the ends of the others. */
+static void nonreflecting_ghost(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, int side, long long cell,
+                                double* Ug) {
+    const int isR = side_high(side), axis = side_axis(side);
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const int n3[3] = {nx, ny, nz};
+    const int ijk[3] = {(int)(cell % nx), (int)((cell / nx) % ny), (int)(cell / ((long long)nx * ny))};
+    int use[3] = {0, 0, 0};
+    double Wlo[3][NV], Whi[3][NV], span[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < 3; ++t) {
+        if (t == axis || n3[t] == 1) continue;
+        const int wrap = c->bc[2 * t] == FVG_BC_PERIODIC && c->bc[2 * t + 1] == FVG_BC_PERIODIC;
+        int lo = ijk[t] - 1, hi = ijk[t] + 1;
+        double sp = 2.0;
+        if (wrap) {
+            lo = (lo + n3[t]) % n3[t];
+            hi = hi % n3[t];
+        } else {
+            if (lo < 0) {
+                lo = 0;
+                sp -= 1.0;
+            }
+            if (hi >= n3[t]) {
+                hi = n3[t] - 1;
+                sp -= 1.0;
+            }
+            if (sp <= 0.0) continue;
+        }
+        int il[3] = {ijk[0], ijk[1], ijk[2]}, ih[3] = {ijk[0], ijk[1], ijk[2]};
+        il[t] = lo;
+        ih[t] = hi;
+        use[t] = 1;
+        c2p(e, U + NV * CELL(m, il[0], il[1], il[2]), Wlo[t], NULL);
+        c2p(e, U + NV * CELL(m, ih[0], ih[1], ih[2]), Whi[t], NULL);
+        span[t] = sp;
+    }
+    const double hs[3] = {m->dx, m->dy, m->dz};
+    const double length[3] = {c->Lx, c->Ly, c->Lz};
+    double Wi[NV], Wg[NV];
+    c2p(e, U + NV * cell, Wi, NULL);
+    for (int q = 0; q < NV; ++q) Wg[q] = Wi[q];
+    const double s = isR ? 1.0 : -1.0;
+    const int qv = kMom[axis];
+    const double u_out = s * Wi[qv];
+    const double cs = sound_speed(e, Wi[3], Wi[0]);
+    const double M = smin(smax(u_out / cs, -1.0), 1.0);
+    if (M >= 1.0) {
+        p2c(e, Wg, Ug);
+        return;
+    }
+    const double h = hs[axis];
+    const double K = c->nrbc_sigma * cs * (1.0 - M * M) / length[axis];
+    const double beta = c->nrbc_beta >= 0.0 ? c->nrbc_beta : fabs(M);
+    double T1 = 0.0;
+    if (beta < 1.0)
+        for (int t = 0; t < 3; ++t) {
+            if (!use[t]) continue;
+            const double* Wl = Wlo[t];
+            const double* Wh = Whi[t];
+            const double ht = span[t] * hs[t];
+            const int qt = kMom[t];
+            T1 += Wi[qt] * (Wh[3] - Wl[3]) / ht + Wi[0] * cs * cs * (Wh[qt] - Wl[qt]) / ht -
+                  Wi[0] * cs * s * Wi[qt] * (Wh[qv] - Wl[qv]) / ht;
+        }
+    const double dw1 = -h * (K * (Wi[3] - c->p_out) - (1.0 - beta) * T1) / (cs - u_out);
+    Wg[3] = Wi[3] + 0.5 * dw1;
+    Wg[qv] = Wi[qv] - s * dw1 / (2.0 * Wi[0] * cs);
+    Wg[0] = Wi[0] + 0.5 * dw1 / (cs * cs);
+    p2c(e, Wg, Ug);
+}
+
+/* apply_boundary_conditions + apply_side (Boundaries.cpp:54-104, 259-371): periodic, ghost_state and non-reflecting
+ * sides; ut: the mass-flow targets of massflow_targets. */
 static int apply_boundary_conditions(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, double* UL, double* UR,
-                                     int interior) {
+                                     int interior, const double ut[6]) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
     if (interior) {
 #pragma omp parallel for schedule(static) num_threads(NT)
@@ -453,7 +554,9 @@ static int apply_boundary_conditions(const fvg_config* c, const Mesh* m, const E
                 default: opp = cell - (long long)(nz - 1) * ny * nx; break;
                 }
                 memcpy(Ug, U + NV * opp, sizeof Ug);
-            } else if (ghost_state(c, e, side, Ui, Ug) != 0) {
+            } else if (bc == FVG_BC_NONREFLECTING) {
+                nonreflecting_ghost(c, m, e, U, side, cell, Ug);
+            } else if (ghost_state(c, e, side, Ui, Ug, ut[side]) != 0) {
                 return -1;
             }
             if (isR) {
@@ -505,7 +608,7 @@ typedef struct {
  * position (i, j, k) of the padded cell: periodic wrap or the ghost slab layer (fill_ghost_slab,
  * Boundaries.cpp:202-228: layer l is the ghost of the interior cell l cells from the boundary). */
 static int halo_state(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, int side, int wrap, int i, int j, int k,
-                      double* out) {
+                      double* out, const double ut[6]) {
     const int axis = side_axis(side), hi = side_high(side);
     const int n3[3] = {m->nx, m->ny, m->nz};
     int idx[3] = {i, j, k};
@@ -517,7 +620,7 @@ static int halo_state(const fvg_config* c, const Mesh* m, const Eos* e, const do
         return 0;
     }
     idx[axis] = hi ? n3[axis] - 1 - l : l;
-    return ghost_state(c, e, side, U + NV * CELL(m, idx[0], idx[1], idx[2]), out);
+    return ghost_state(c, e, side, U + NV * CELL(m, idx[0], idx[1], idx[2]), out, ut[side]);
 }
 
 static inline int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
@@ -525,7 +628,7 @@ static inline int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i)
 /* pad_primitives (Schemes.cpp:915-969): clamp-replicated primitive state (prim = 0: the conservative state as it
  * is), with the halo slabs in place. */
 static int pad_primitives(const fvg_config* c, const Mesh* m, const Eos* e, const double* U, int pad, const int has[6],
-                          const int wrap[3], Padded* P, int prim) {
+                          const int wrap[3], Padded* P, int prim, const double ut[6]) {
     const int nx = m->nx, ny = m->ny, nz = m->nz;
     P->pad = pad;
     P->zp = m->is3d ? pad : 0;
@@ -550,7 +653,7 @@ static int pad_primitives(const fvg_config* c, const Mesh* m, const Eos* e, cons
                 const int t = axis == 0 ? i : (axis == 1 ? j : k);
                 const int side = 2 * axis + (t >= 0);
                 if (has[side]) {
-                    if (halo_state(c, m, e, U, side, wrap[axis], i, j, k, Uc) != 0) bad = 1;
+                    if (halo_state(c, m, e, U, side, wrap[axis], i, j, k, Uc, ut) != 0) bad = 1;
                     src = Uc;
                 }
             }
@@ -993,19 +1096,19 @@ static void hancock_wall_ghosts(const fvg_config* c, const Mesh* m, const Eos* e
     }
     for (int k = 0; k < nz; ++k) {
         for (int j = 0; j < ny; ++j) {
-            if (wall[0]) ghost_state(c, e, 0, UR + NV * XF(m, 0, j, k), UL + NV * XF(m, 0, j, k));
-            if (wall[1]) ghost_state(c, e, 1, UL + NV * XF(m, nx, j, k), UR + NV * XF(m, nx, j, k));
+            if (wall[0]) ghost_state(c, e, 0, UR + NV * XF(m, 0, j, k), UL + NV * XF(m, 0, j, k), 0.0);
+            if (wall[1]) ghost_state(c, e, 1, UL + NV * XF(m, nx, j, k), UR + NV * XF(m, nx, j, k), 0.0);
         }
         for (int i = 0; i < nx; ++i) {
-            if (wall[2]) ghost_state(c, e, 2, UR + NV * YF(m, i, 0, k), UL + NV * YF(m, i, 0, k));
-            if (wall[3]) ghost_state(c, e, 3, UL + NV * YF(m, i, ny, k), UR + NV * YF(m, i, ny, k));
+            if (wall[2]) ghost_state(c, e, 2, UR + NV * YF(m, i, 0, k), UL + NV * YF(m, i, 0, k), 0.0);
+            if (wall[3]) ghost_state(c, e, 3, UL + NV * YF(m, i, ny, k), UR + NV * YF(m, i, ny, k), 0.0);
         }
     }
     if (m->is3d)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
-                if (wall[4]) ghost_state(c, e, 4, UR + NV * ZF(m, i, j, 0), UL + NV * ZF(m, i, j, 0));
-                if (wall[5]) ghost_state(c, e, 5, UL + NV * ZF(m, i, j, nz), UR + NV * ZF(m, i, j, nz));
+                if (wall[4]) ghost_state(c, e, 4, UR + NV * ZF(m, i, j, 0), UL + NV * ZF(m, i, j, 0), 0.0);
+                if (wall[5]) ghost_state(c, e, 5, UL + NV * ZF(m, i, j, nz), UR + NV * ZF(m, i, j, nz), 0.0);
             }
 }
 
@@ -1600,8 +1703,6 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
     const int recon = c->reconstruction != FVG_RECON_NONE;
     const int fourth = c->flux_quadrature == FVG_QUAD_FOURTH;
     if (fourth && c->reconstruction != FVG_RECON_WENO3 && c->reconstruction != FVG_RECON_WENO5) return -3;
-    for (int s = 0; s < 6; ++s)
-        if (c->bc[s] == FVG_BC_NONREFLECTING || c->bc[s] == FVG_BC_MASSFLOW_INLET) return -3;
     double* UL = (double*)malloc(sizeof(double) * NV * (size_t)m.nf);
     double* UR = (double*)malloc(sizeof(double) * NV * (size_t)m.nf);
     double* F = (double*)malloc(sizeof(double) * NV * (size_t)m.nf);
@@ -1609,14 +1710,16 @@ int fvo_residual(const fvg_config* c, const double* U, double* R, double half_dt
         free(UL); free(UR); free(F);
         return -2;
     }
-    int rc = apply_boundary_conditions(c, &m, e, U, UL, UR, !recon);
+    double ut[6];
+    massflow_targets(c, &m, e, U, ut);
+    int rc = apply_boundary_conditions(c, &m, e, U, UL, UR, !recon, ut);
     long long rev = 0;
     if (rc == 0 && recon) {
         const int Dh = c->reconstruction == FVG_RECON_WENO5 ? 3 : 2;
         int has[6], wrap[3];
         halo_sides(c, &m, Dh, has, wrap);
         Padded P;
-        rc = pad_primitives(c, &m, e, U, Dh, has, wrap, &P, !fourth);
+        rc = pad_primitives(c, &m, e, U, Dh, has, wrap, &P, !fourth, ut);
         if (rc == 0) {
             if (fourth) reconstruct_conserved(c, &m, e, &P, has, UL, UR);
             else reconstruct_faces(c, &m, e, &P, has, UL, UR);

@@ -113,6 +113,25 @@ def matrix():
     out.append(("2d-thin-weno3-periodic-n2", base_keys(9, 2, reconstruction="weno3"), "shock"))
     out.append(("2d-thin-hancock-walls-bo2-n2", base_keys(6, 2, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **walls),
                 "shock"))
+    # a mass-flow inlet (the side's mean density sets the inflow velocity) and a non-reflecting outlet (characteristic
+    # ghost relaxed towards p_out, transverse terms), in 2-D and 3-D, every scheme family
+    io = dict(bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_ymin="wall", bc_ymax="no_slip", u_in="0.5", v_in="0.1",
+              rho_in="1.2", p_out="0.9")
+    io3 = dict(io, bc_zmin="periodic", bc_zmax="periodic")
+    for bo in (1, 2):
+        out.append((f"2d-inflow-nrbc-muscl-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, **io), "shock"))
+    out.append(("2d-inflow-nrbc-weno5", base_keys(12, 10, periodic=False, boundary_order=2, reconstruction="weno5", **io), "smooth"))
+    out.append(("2d-inflow-nrbc-hancock", base_keys(12, 10, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **io), "shock"))
+    out.append(("2d-inflow-nrbc-visc", base_keys(12, 10, periodic=False, boundary_order=2, use_viscous="true", mu="0.01", **io), "smooth"))
+    out.append(("2d-nrbc-beta-sigma", base_keys(12, 10, periodic=False, boundary_order=2, nrbc_beta="0.5", nrbc_sigma="0.6", **io), "shock"))
+    out.append(("2d-inflow-nrbc-y-periodic-x", base_keys(12, 10, periodic=False, bc_xmin="periodic", bc_xmax="periodic",
+                                                         bc_ymin="massflow_inlet", bc_ymax="nonreflecting", u_in="0.3", rho_in="0.9",
+                                                         p_out="1.1"), "smooth"))
+    out.append(("3d-inflow-nrbc-weno5-visc", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno5", use_viscous="true",
+                                                       mu="0.01", **io3), "smooth"))
+    out.append(("3d-inflow-nrbc-z-walls", base_keys(8, 9, 7, periodic=False, boundary_order=1, **dict(io, bc_zmin="wall",
+                                                                                                       bc_zmax="nonreflecting")), "smooth"))
+    out.append(("3d-inflow-nrbc-hancock", base_keys(8, 9, 7, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **io3), "shock"))
     return out
 
 
@@ -138,6 +157,10 @@ def fourth_matrix():
         ("3d-walls-weno5-fourth-bo1", base_keys(8, 9, 7, periodic=False, boundary_order=1, reconstruction="weno5", **walls3, **F), "shock"),
         ("3d-thin-weno5-fourth-n2", base_keys(6, 4, 2, reconstruction="weno5", **F), "shock"),
         ("2d-euler-weno5-fourth", base_keys(12, 10, reconstruction="weno5", time_scheme="Explicit Euler", **F), "smooth"),
+        ("3d-inflow-nrbc-weno5-fourth", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno5",
+                                                  **dict(walls, bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_zmin="periodic",
+                                                         bc_zmax="periodic", u_in="0.5", v_in="0.1", rho_in="1.2", p_out="0.9"), **F),
+         "shock"),
     ]
 
 

@@ -12,7 +12,7 @@ holds
 The reference is also run here from U0 and must land on U_ref within the reference's tolerance 1e-8
 (regression.py:305); the script stops if it does not.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
@@ -90,6 +90,13 @@ REGRESSION = {
     "cavity3d_weno5_b2": ("Lid Driven Cavity", dict(nx=12, ny=12, nz=8, Lz=1.0, t_end=0.1, bc_zmin="no_slip",
                                                     bc_zmax="no_slip", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3",
                                                     boundary_order=2)),
+    # non-reflecting outlets (regression.py:174-182)
+    "vortex_nrbc": ("Isentropic Vortex", dict(nx=40, ny=48, t_end=1.0, reconstruction="muscl", bc_xmin="transmissive",
+                                              bc_xmax="nonreflecting")),
+    "vortex_nrbc_xdecomp": ("Isentropic Vortex", dict(nx=64, ny=6, t_end=0.5, bc_xmin="nonreflecting", bc_xmax="nonreflecting",
+                                                      nrbc_sigma=1.0)),
+    "tgv3d_nrbc": ("Taylor-Green Vortex 3D", dict(nx=16, ny=16, nz=8, t_end=0.3, bc_xmin="transmissive", bc_xmax="nonreflecting",
+                                                  bc_zmin="nonreflecting", bc_zmax="nonreflecting")),
 }
 
 
@@ -102,9 +109,9 @@ def keys_for(name):
     return case, {k: str(v) for k, v in keys.items()}
 
 
-def main():
+def main(names=None):
     ref = Reference()
-    for name in REGRESSION:
+    for name in names or REGRESSION:
         case, keys = keys_for(name)
         run = ref.create(case, **keys)
         U0 = run.initial_field()
@@ -123,4 +130,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
