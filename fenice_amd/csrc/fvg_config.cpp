@@ -188,6 +188,35 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
         if (lv == "fourth") return (c->flux_quadrature = FVG_QUAD_FOURTH), FVG_OK;
         return fail(err, n, "flux_quadrature must be midpoint or fourth; got '" + lv + "'");
     }
+    // The reference's other keys (Params.cpp:98-223). The run control of the host driver (output, monitor, restart,
+    // threads) is accepted and has no effect here; the parameters of the immersed body and of AMR are inert while those
+    // features are off; a feature outside the uniform-grid path is refused when it is switched on.
+    static const char* host_keys[] = {"test_case", "save_interval", "write_interval_time", "monitor_interval", "resume",
+                                      "dump_state", "vtk_format", "n_cpus"};
+    for (const char* h : host_keys)
+        if (key == h) return FVG_OK;
+    auto refused = [&](const std::string& what) {
+        if (err && n) std::snprintf(err, n, "%s", (key + " = " + v + ": " + what + " is outside the uniform-grid device path").c_str());
+        return int(FVG_EUNSUPPORTED);
+    };
+    static const char* switches[][2] = {{"use_bodyforce", "a body force"}, {"use_ibm", "the immersed boundary"}, {"use_amr", "AMR"},
+                                        {"use_mpi", "MPI decomposition"}, {"positivity_fallback", "the positivity fallback"}};
+    for (const auto& sw : switches) {
+        if (key != sw[0]) continue;
+        if (!parse_flag(v, k)) return fail(err, n, "'" + key + "': not a flag: '" + v + "'");
+        return k ? refused(sw[1]) : int(FVG_OK);
+    }
+    if (key == "source_rho" || key == "source_mx" || key == "source_my" || key == "source_mz" || key == "source_E")
+        // a source term is on for an expression or a non-zero number (Params.cpp:282-286)
+        return parse_num(v, d) && d == 0.0 ? int(FVG_OK) : refused("a source term");
+    if (key == "gx" || key == "gy" || key == "gz" || key == "reynolds") {
+        if (!parse_num(v, d)) return fail(err, n, "'" + key + "': not a number: '" + v + "'");
+        if (key == "reynolds" && d != 0.0) return refused("mu from the Reynolds number of an immersed body");
+        return FVG_OK;   // gravity: used only with use_bodyforce
+    }
+    if (key == "u_in_table" || key == "v_in_table" || key == "rho_in_table" || key == "p_out_table")
+        return trim(v).empty() ? int(FVG_OK) : refused("a boundary value table");
+    if (key.compare(0, 4, "ibm_") == 0 || key.compare(0, 4, "amr_") == 0) return FVG_OK;
     static const char* bc_keys[6] = {"bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax", "bc_zmin", "bc_zmax"};
     for (int s = 0; s < 6; ++s) {
         if (key != bc_keys[s]) continue;

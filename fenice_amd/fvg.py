@@ -156,7 +156,7 @@ def config_from_keys(keys: dict) -> Config:
             v = "true" if v else "false"
         rc = L.fvg_config_set(C.byref(c), str(k).encode(), str(v).encode(), err, 512)
         if rc != OK:
-            raise InvalidArgument(err.value.decode())
+            raise _ERR.get(rc, InvalidArgument)(err.value.decode())
     rc = L.fvg_config_finalize(C.byref(c), err, 512)
     if rc != OK:
         raise InvalidArgument(err.value.decode())

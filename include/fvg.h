@@ -111,7 +111,11 @@ void fvg_config_defaults(fvg_config* cfg);
  * ("HLLC (Low-Mach Thornber)", "lowmach", "Van Leer", "SSP Runge-Kutta 3", "periodic", ...). Replaces
  * SimParams::from + parse_riemann / parse_limiter / parse_reconstruction / parse_time_scheme / parse_bc
  * (Params.cpp, Schemes.cpp:77-102, TimeScheme.cpp:78-85, Boundaries.cpp:15-27).
- * Returns FVG_EINVAL with a message in err (may be NULL) for an unknown key or label. */
+ * The reference's other keys are accepted: run control of its driver (test_case, save_interval, monitor_interval,
+ * resume, dump_state, ...) has no effect, immersed-body and AMR parameters are inert; switching on a feature outside
+ * this path (use_ibm, use_amr, use_mpi, use_bodyforce, positivity_fallback, a source term, reynolds, a boundary
+ * value table) returns FVG_EUNSUPPORTED. Returns FVG_EINVAL with a message in err (may be NULL) for an unknown key
+ * or label. */
 int fvg_config_set(fvg_config* cfg, const char* key, const char* value, char* err, size_t err_len);
 /* Resolve time_scheme = auto (Params.cpp: MUSCL-Hancock for inviscid MUSCL, SSP-RK2 for viscous MUSCL,
  * Explicit Euler otherwise) and check sizes. */

@@ -100,6 +100,25 @@ def test_invalid_keys_raise_value_error(keys, msg):
         fvg.config_from_keys(keys)
 
 
+def test_reference_case_keys_outside_the_path():
+    # a whole case file's keys (Params.cpp parameter_table): run control and inert feature parameters are accepted,
+    # a switched-on feature outside the path is refused as unsupported
+    keys = dict(test_case="Sod Shock Tube", nx=40, ny=4, bc_xmin="transmissive", bc_xmax="transmissive", bc_ymin="wall",
+                bc_ymax="wall", save_interval=10, write_interval_time="0.01", monitor_interval=5, resume="false", dump_state="",
+                vtk_format="binary", n_cpus=4, use_bodyforce="false", gx="0", gy="-9.81", gz="0", source_rho="0", source_mx="0.0",
+                use_ibm="false", ibm_shape="Cylinder", ibm_radius="0.1", use_amr="no", amr_max_level=2, use_mpi="off",
+                positivity_fallback="false", reynolds="0", u_in_table="", p_out_table="")
+    c = fvg.config_from_keys(keys)
+    assert c.nx == 40 and c.bc[0] == BC_TRANSMISSIVE
+    for k, v in [("use_ibm", "true"), ("use_amr", "1"), ("use_mpi", "yes"), ("use_bodyforce", "on"),
+                 ("positivity_fallback", "true"), ("source_rho", "0.1"), ("source_E", "sin(x)"), ("reynolds", "100"),
+                 ("u_in_table", "inflow.csv")]:
+        with pytest.raises(fvg.Unsupported, match="outside the uniform-grid device path"):
+            fvg.config_from_keys(dict(nx=8, ny=8, **{k: v}))
+    with pytest.raises(ValueError, match="not a flag"):
+        fvg.config_from_keys(dict(nx=8, ny=8, use_ibm="maybe"))
+
+
 def test_dt_length():
     c = fvg.config_from_keys(dict(nx=8, ny=4, Lx=1.0, Ly=1.0))
     assert fvg.lib().fvg_dt_length(C.byref(c)) == 0.125
