@@ -23,7 +23,7 @@ template <int NV> __global__ void k_hancock(KP);
 template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV, int WHAT> __global__ void k_totals_partial(const double*, long long, double*);
-template <int NV, int D> __global__ void k_faces_learned(KP);
+template <int NV, int D, int USE> __global__ void k_faces_learned(KP);
 template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
 template <int NV, int D> __global__ void k_audit(KP);
 template <int NV, int D> __global__ void k_samples(KP, float*, float*);
@@ -88,6 +88,7 @@ struct fvg_ctx {
     double fam_ms[FAM_N] = {};
     long long fam_n[FAM_N] = {};
     bool learned = false;              // learned troubled-face indicator
+    unsigned char* ind_kept[3] = {nullptr, nullptr, nullptr};   // KP::ind_kept
     MlpDev* mlp = nullptr;
     IndState* ind = nullptr;
     bool graphs = true;                // fvg_advance replays a captured graph of steps
@@ -209,8 +210,10 @@ int enqueue_massflow(fvg_ctx* c, const KP& p) {
     return rc;
 }
 
+// reuse (learned indicator, Runge-Kutta stages 2 and 3): the faces take the decisions the network made on the step's
+// first stage instead of running it again
 template <int NV, typename T>
-int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
+int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit, bool reuse) {
     const KP p = make_kp(c, Uin);
     int rc;
     if ((rc = enqueue_massflow(c, p))) return rc;
@@ -220,16 +223,21 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit) {
         if (mode == MODE_LEARNED) {
 #define LFACES(KERNEL, FAM, D, GRID, BLOCK) \
     launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(GRID), dim3(BLOCK), 0, c->stream, p); })
+#define LEARNED(FAM, D)                                                                                                         \
+    launch(c, FAM, [&] {                                                                                                      \
+        if (reuse) hipLaunchKernelGGL((k_faces_learned<NV, D, LF_KEPT>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p); \
+        else hipLaunchKernelGGL((k_faces_learned<NV, D, LF_STORE>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p);     \
+    })
             if (audit) {
                 if ((rc = LFACES(k_audit, FAM_AUDIT, 0, grid_for(c->nfa[0]), kBlock)) || (rc = LFACES(k_audit, FAM_AUDIT, 1, grid_for(c->nfa[1]), kBlock)))
                     return rc;
                 if constexpr (NV == 5)
                     if ((rc = LFACES(k_audit, FAM_AUDIT, 2, grid_for(c->nfa[2]), kBlock))) return rc;
             }
-            if ((rc = LFACES(k_faces_learned, FAM_FACES_X, 0, faces_grid(c, 0), faces_block(c, 0))) || (rc = LFACES(k_faces_learned, FAM_FACES_Y, 1, faces_grid(c, 1), faces_block(c, 1))))
-                return rc;
+            if ((rc = LEARNED(FAM_FACES_X, 0)) || (rc = LEARNED(FAM_FACES_Y, 1))) return rc;
             if constexpr (NV == 5)
-                if ((rc = LFACES(k_faces_learned, FAM_FACES_Z, 2, faces_grid(c, 2), faces_block(c, 2)))) return rc;
+                if ((rc = LEARNED(FAM_FACES_Z, 2))) return rc;
+#undef LEARNED
 #undef LFACES
             return FVG_OK;
         }
@@ -299,12 +307,12 @@ int enqueue_hancock_residual(fvg_ctx* c, const double* Uin) {
     return enqueue_hancock_faces<NV>(c, p);
 }
 
-int enqueue_residual(fvg_ctx* c, const double* Uin, int mode, bool audit = false) {
+int enqueue_residual(fvg_ctx* c, const double* Uin, int mode, bool audit = false, bool reuse = false) {
     if (mode == MODE_HANCOCK)
         return c->NV == 5 ? enqueue_hancock_residual<5>(c, Uin) : enqueue_hancock_residual<4>(c, Uin);
     if (c->fp32)
-        return c->NV == 5 ? enqueue_residual_nv<5, float>(c, Uin, mode, audit) : enqueue_residual_nv<4, float>(c, Uin, mode, audit);
-    return c->NV == 5 ? enqueue_residual_nv<5, double>(c, Uin, mode, audit) : enqueue_residual_nv<4, double>(c, Uin, mode, audit);
+        return c->NV == 5 ? enqueue_residual_nv<5, float>(c, Uin, mode, audit, reuse) : enqueue_residual_nv<4, float>(c, Uin, mode, audit, reuse);
+    return c->NV == 5 ? enqueue_residual_nv<5, double>(c, Uin, mode, audit, reuse) : enqueue_residual_nv<4, double>(c, Uin, mode, audit, reuse);
 }
 
 int enqueue_indicator(fvg_ctx* c, int op) {
@@ -349,9 +357,9 @@ int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const 
         rc = launch(c, FAM_FACES_X + d, [&] {
             const dim3 g(grid_for(c->nfa[d])), b(kBlock);
             if constexpr (std::is_same<T, float>::value) {
-                if (mode == MODE_LEARNED) {
-                    if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0>), g, b, 0, c->stream, p);
-                    else hipLaunchKernelGGL((k_faces_learned<NV, 1>), g, b, 0, c->stream, p);
+                if (mode == MODE_LEARNED) {   // stages 2-3: the first stage's decisions
+                    if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0, LF_KEPT>), g, b, 0, c->stream, p);
+                    else hipLaunchKernelGGL((k_faces_learned<NV, 1, LF_KEPT>), g, b, 0, c->stream, p);
                     return;
                 }
             }
@@ -448,7 +456,7 @@ int enqueue_step(fvg_ctx* c) {
         if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
         return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
-    if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
+    if ((rc = enqueue_residual(c, c->Ut, mode, false, c->learned))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
     if (ts == FVG_TS_SSPRK2) {
         if ((rc = enqueue_update(c, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
@@ -456,7 +464,7 @@ int enqueue_step(fvg_ctx* c) {
         return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_RK3_2, c->U, c->Ut, c->Ut))) return rc;
-    if ((rc = enqueue_residual(c, c->Ut, mode))) return rc;
+    if ((rc = enqueue_residual(c, c->Ut, mode, false, c->learned))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
     if ((rc = enqueue_update(c, UPD_RK3_3, c->U, c->Ut, c->U))) return rc;
     if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
@@ -518,7 +526,7 @@ void free_ctx(fvg_ctx* c) {
     hipFree(c->ind);
     void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,         c->HS,  c->Fc[0], c->Fc[1], c->Fc[2],
                     c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd, c->Wc,    c->q4_UL, c->q4_UR,
-                    c->q4_Fp, c->q4_good, c->q4_pt, c->nrbc};
+                    c->q4_Fp, c->q4_good, c->q4_pt, c->nrbc, c->ind_kept[0], c->ind_kept[1], c->ind_kept[2]};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->st) hipFree(c->st);
@@ -769,6 +777,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     if (c->learned) {
         rc = rc ? rc : checked_alloc(c, &c->mlp, 1, sizeof(MlpDev));
         rc = rc ? rc : checked_alloc(c, &c->ind, 1, sizeof(IndState));
+        for (int d = 0; d < ND; ++d) rc = rc ? rc : checked_alloc(c, &c->ind_kept[d], (size_t)c->nfa[d], 1);
         if (!rc) {   // until weights are set every face is troubled (WENO5-Z everywhere)
             MlpDev m{};
             m.thr = -std::numeric_limits<float>::max();
@@ -817,6 +826,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     p.q4_pt = c->q4_pt;
     p.q4_n = q4_n;
     p.mlp = c->mlp;
+    for (int d = 0; d < 3; ++d) p.ind_kept[d] = c->ind_kept[d];
     p.ind = c->ind;
     p.ind_audit = cfg.indicator_audit;
     p.ind_label = (float)cfg.indicator_label;

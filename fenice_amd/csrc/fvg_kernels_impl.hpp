@@ -923,25 +923,44 @@ __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
 
 // States and flux of face (i, j, k) normal to D through the learned indicator. Every lane of the wavefront calls
 // it (the network runs on the matrix cores); lanes with active = false only take part in the network.
-template <int NV, int D>
-__device__ __forceinline__ void learned_face(const KP& p, bool active, int i, int j, int k, int t, int n, float* F, float& sm) {
+// USE: the network decides and its decision is kept for the step's later stages (LF_STORE, the first Runge-Kutta
+// stage), the kept decision of face f is used (LF_KEPT, stages 2 and 3), or the network decides without keeping it
+// (LF_NET, the fused last-axis kernel)
+template <int NV, int D> __device__ __forceinline__ unsigned char* kept_of(const KP& p) {
+    return D == 0 ? p.ind_kept[0] : (D == 1 ? p.ind_kept[1] : p.ind_kept[2]);
+}
+template <int NV, int D, int USE>
+__device__ __forceinline__ void learned_face(const KP& p, bool active, long long f, int i, int j, int k, int t, int n, float* F,
+                                             float& sm) {
     const bool upgrade = active && face_upgrade<NV, D>(p, t);
     float c[6][NV];
-    float x[16];
+    bool troubled = true;
+    if constexpr (USE == LF_KEPT) {
+        if (upgrade) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = 0.0f;
-    if (upgrade) {
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+            troubled = kept_of<NV, D>(p)[f] != 0;
+        }
+    } else {
+        float x[16];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
-        face_features<NV>(c, float(p.weno_eps), x);
+        for (int q = 0; q < 16; ++q) x[q] = 0.0f;
+        if (upgrade) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+            face_features<NV>(c, float(p.weno_eps), x);
+        }
+        const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
+        troubled = z > p.mlp->thr;
+        if constexpr (USE == LF_STORE)
+            if (upgrade) kept_of<NV, D>(p)[f] = troubled ? 1 : 0;
     }
-    const float z = mlp_logit_wave(p.mlp, x);
     if (!active) return;
     const bool lo = t == 0, hi = t == n;
     const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
     float UL[NV], UR[NV];
     if (upgrade) {
-        const bool smooth = z <= p.mlp->thr && !p.ind->off;
+        const bool smooth = !troubled && !p.ind->off;
         float wl[NV], wr[NV];
         const float we = float(p.weno_eps);
         if (smooth) {
@@ -973,7 +992,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, int i, in
     face_flux<NV>(p.rs, UL, UR, D, F, sm);
 }
 
-template <int NV, int D>
+template <int NV, int D, int USE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_faces_learned(KP p) {
     if (p.st->done) return;   // uniform: the whole grid leaves
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
@@ -992,7 +1011,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     const int n = ncell(p, D);
     float F[NV], sm = 0.0f;
-    learned_face<NV, D>(p, active, i, j, k, t, n, F, sm);
+    learned_face<NV, D, USE>(p, active, f, i, j, k, t, n, F, sm);
     double local = 0.0;
     if (active) {
         if (double(sm) > local) local = double(sm);
@@ -1170,7 +1189,7 @@ __device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j
     const int n = ncell(p, D);
     T sm = T(0);
     if constexpr (MODE == MODE_LEARNED) {
-        learned_face<NV, D>(p, active, i, j, k, kf, n, Fc, sm);
+        learned_face<NV, D, LF_NET>(p, active, 0, i, j, k, kf, n, Fc, sm);
     } else {
         if (!active) return;
         T UL[NV], UR[NV];
