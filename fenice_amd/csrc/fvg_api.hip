@@ -416,23 +416,30 @@ int plain_mode(const fvg_ctx* c) {
 }
 
 // One step of the configured scheme, enqueued (TimeScheme.cpp:110-319).
-int enqueue_step(fvg_ctx* c) {
+// link (MUSCL-Hancock with a lagged speed only, for steps enqueued back to back): LINK_AFTER, the previous step's end
+// was left to this step's first control launch; LINK_BEFORE, leave this step's end to the next step's.
+enum { LINK_AFTER = 1, LINK_BEFORE = 2 };
+int enqueue_step(fvg_ctx* c, int link = 0) {
     int rc;
     const int ts = c->cfg.time_scheme;
-    if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
     if (ts == FVG_TS_HANCOCK) {
         const int mode = c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_HANCOCK;
         if (!c->lagged_positive) {   // no wave speed yet: one plain residual gives it (TimeScheme.cpp:298-302)
+            link = 0;
+            if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
             if ((rc = enqueue_residual(c, c->U, plain_mode(c)))) return rc;
             if ((rc = enqueue_control(c, CTL_SEED, ts))) return rc;
+            if ((rc = enqueue_control(c, CTL_HANCOCK_DT, ts))) return rc;
+        } else if ((rc = enqueue_control(c, (link & LINK_AFTER) ? CTL_END_BEGIN_HANCOCK : CTL_BEGIN_HANCOCK, ts))) {
+            return rc;
         }
-        if ((rc = enqueue_control(c, CTL_HANCOCK_DT, ts))) return rc;
         if ((rc = enqueue_residual(c, c->U, mode))) return rc;
         if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
-        if ((rc = enqueue_control(c, CTL_END_HANCOCK, ts))) return rc;
+        if (!(link & LINK_BEFORE) && (rc = enqueue_control(c, CTL_END_HANCOCK, ts))) return rc;
         c->lagged_positive = true;
         return FVG_OK;
     }
+    if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
     const int mode = plain_mode(c);
     if (c->learned && (rc = enqueue_indicator(c, CTL_AUDIT_BEGIN))) return rc;
     if ((rc = enqueue_residual(c, c->U, mode, c->learned))) return rc;
@@ -480,7 +487,10 @@ int ensure_graph(fvg_ctx* c) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = FVG_OK;
-    for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(c);
+    // MUSCL-Hancock: the end of each step and the start of the next share one control launch
+    const bool hancock = c->cfg.time_scheme == FVG_TS_HANCOCK && c->lagged_positive;
+    for (int k = 0; k < kGraphSteps && !rc; ++k)
+        rc = enqueue_step(c, hancock ? (k > 0 ? LINK_AFTER : 0) | (k + 1 < kGraphSteps ? LINK_BEFORE : 0) : 0);
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc || e != hipSuccess) {
         if (g) hipGraphDestroy(g);

@@ -106,6 +106,32 @@ __global__ void k_control(KP p, int op, int arg) {
         }
     }
     if (threadIdx.x != 0) return;
+    // MUSCL-Hancock steps chained in one launch: the end of a step and the start of the next (the same operations, in
+    // the same order, as the separate CTL_END_HANCOCK, CTL_BEGIN_STEP and CTL_HANCOCK_DT launches)
+    if (op == CTL_END_BEGIN_HANCOCK) {
+        if (s->done) return;
+        const double S = stage_speed(s, p.viscous);
+        s->s_conv = 0;
+        s->s_visc = 0;
+        s->lagged = S;
+        s->s_step = S;
+        s->half_dt = 0.0;
+        end_step(s);
+        op = CTL_BEGIN_HANCOCK;
+    }
+    if (op == CTL_BEGIN_HANCOCK) {
+        if (s->done) return;
+        if (!(s->t < s->t_end) || (s->max_steps > 0 && s->steps >= s->max_steps)) {
+            s->done = 1;
+            return;
+        }
+        s->t_max = s->t_end < s->t_max ? s->t_end : s->t_max;
+        s->s_conv = 0;
+        s->s_visc = 0;
+        s->dt = clamp_dt(s, p.cfl * p.Ldt / (s->lagged + kDtEps));
+        s->half_dt = 0.5 * s->dt;
+        return;
+    }
     if (op == CTL_BEGIN_STEP) {
         if (s->done) return;
         if (!(s->t < s->t_end) || (s->max_steps > 0 && s->steps >= s->max_steps)) {

@@ -110,6 +110,8 @@ enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3,
 enum { LF_STORE = 0, LF_KEPT = 1, LF_NET = 2 };   // use of the learned indicator's decisions (learned_face)
 enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
 enum { CTL_BEGIN_STEP = 0, CTL_STAGE1_DT = 1, CTL_STAGE_S = 2, CTL_END_RK = 3, CTL_SEED = 4, CTL_HANCOCK_DT = 5, CTL_END_HANCOCK = 6,
-       CTL_RESIDUAL_S = 7, CTL_AUDIT_BEGIN = 8, CTL_AUDIT_END = 9 };
+       CTL_RESIDUAL_S = 7, CTL_AUDIT_BEGIN = 8, CTL_AUDIT_END = 9,
+       CTL_BEGIN_HANCOCK = 10,       // CTL_BEGIN_STEP then CTL_HANCOCK_DT, in one launch
+       CTL_END_BEGIN_HANCOCK = 11 }; // CTL_END_HANCOCK of the previous step, then CTL_BEGIN_HANCOCK
 
 }   // namespace fvg
