@@ -107,7 +107,7 @@ constexpr int PXA = 32;   // row alignment of the padded planes in elements (128
 constexpr int PX0 = 32;   // element of a padded row that holds i = 0 (the low x halo sits just before it)
 
 enum { MODE_PLAIN = 0, MODE_HANCOCK = 1, MODE_FIRST_ORDER = 2, MODE_LEARNED = 3, MODE_FOURTH = 4 };
-enum { LF_STORE = 0, LF_KEPT = 1, LF_NET = 2 };   // use of the learned indicator's decisions (learned_face)
+enum { LF_STORE = 0, LF_KEPT = 1 };   // use of the learned indicator's decisions (learned_face)
 enum { UPD_RESIDUAL = 0, UPD_AXPY = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
 enum { CTL_BEGIN_STEP = 0, CTL_STAGE1_DT = 1, CTL_STAGE_S = 2, CTL_END_RK = 3, CTL_SEED = 4, CTL_HANCOCK_DT = 5, CTL_END_HANCOCK = 6,
        CTL_RESIDUAL_S = 7, CTL_AUDIT_BEGIN = 8, CTL_AUDIT_END = 9,

@@ -924,8 +924,7 @@ __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
 // States and flux of face (i, j, k) normal to D through the learned indicator. Every lane of the wavefront calls
 // it (the network runs on the matrix cores); lanes with active = false only take part in the network.
 // USE: the network decides and its decision is kept for the step's later stages (LF_STORE, the first Runge-Kutta
-// stage), the kept decision of face f is used (LF_KEPT, stages 2 and 3), or the network decides without keeping it
-// (LF_NET, the fused last-axis kernel)
+// stage), or the kept decision of face f is used (LF_KEPT, stages 2 and 3, also in the fused last-axis kernel)
 template <int NV, int D> __device__ __forceinline__ unsigned char* kept_of(const KP& p) {
     return D == 0 ? p.ind_kept[0] : (D == 1 ? p.ind_kept[1] : p.ind_kept[2]);
 }
@@ -941,7 +940,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
             for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
             troubled = kept_of<NV, D>(p)[f] != 0;
         }
-    } else {
+    } else {   // LF_STORE
         float x[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) x[q] = 0.0f;
@@ -952,8 +951,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
         }
         const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
         troubled = z > p.mlp->thr;
-        if constexpr (USE == LF_STORE)
-            if (upgrade) kept_of<NV, D>(p)[f] = troubled ? 1 : 0;
+        if (upgrade) kept_of<NV, D>(p)[f] = troubled ? 1 : 0;
     }
     if (!active) return;
     const bool lo = t == 0, hi = t == n;
@@ -1189,7 +1187,9 @@ __device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j
     const int n = ncell(p, D);
     T sm = T(0);
     if constexpr (MODE == MODE_LEARNED) {
-        learned_face<NV, D, LF_NET>(p, active, 0, i, j, k, kf, n, Fc, sm);
+        // stages 2-3: the decisions the network made on the step's first stage, as in the separate face passes
+        const long long f = D == 2 ? ((long long)k * p.ny + j) * p.nx + i : (long long)j * p.nx + i;
+        learned_face<NV, D, LF_KEPT>(p, active, f, i, j, k, kf, n, Fc, sm);
     } else {
         if (!active) return;
         T UL[NV], UR[NV];
