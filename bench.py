@@ -71,9 +71,26 @@ def error_vs_fp64(n: int, flux: str, indicator: str = "none", weights: str = "",
 # CPU baseline (runs in a child process): the reference's own code built from its sources (oracle/_ref), or the
 # oracle where that build is absent, on the same workload, a bounded sample of steps.
 # ---------------------------------------------------------------------------------------------------------------
-def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint") -> dict:
+def cpu_workload(name: str, n: int, quadrature: str = "midpoint"):
+    """(keys, initial state (cells, 5), cells, RK stages per step, most timed steps, description) of a CPU sample."""
+    from fenice_amd.workloads import kelvin_helmholtz, kh_keys, vortex_keys, vortex_state
+
+    def five(U4):
+        U = np.zeros((U4.shape[0], 5))
+        U[:, :4] = U4
+        return U
+    if name == "vortex":
+        return (vortex_keys(n, cfl="0.7"), five(vortex_state(n)), n * n, 1, 200,
+                f"isentropic vortex {n}^2 MUSCL Van Leer+HLLC+MUSCL-Hancock, CFL 0.7")
+    if name == "kh":
+        return kh_keys(n), five(kelvin_helmholtz(n)), n * n, 3, 3, f"Kelvin-Helmholtz {n}^2 WENO5-Z+HLLC+SSP-RK3"
+    return (run_keys(n, "fp64", quadrature), tgv_state(n), n ** 3, 3, 3,
+            f"3-D Taylor-Green {n}^3 WENO5-Z+HLLC+SSP-RK3 NS" + (", fourth-order flux quadrature" if quadrature == "fourth" else ""))
+
+
+def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv") -> dict:
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libfenice_ref.so")
-    U0 = tgv_state(n)
+    keys, U0, cells, nst, cap, desc = cpu_workload(workload, n, quadrature)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     if os.path.exists(ref_so):
         L = C.CDLL(ref_so)
@@ -81,7 +98,6 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint") -
         L.ref_create.restype = C.c_void_p
         L.ref_set_state.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ref_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        keys = run_keys(n, "fp64", quadrature)
         ks = [k.encode() for k in keys]
         vs = [str(v).encode() for v in keys.values()]
         # a run description of our own for the reference's case loader (every key of the run is an override below)
@@ -106,32 +122,33 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint") -
         from oracle_lib import Oracle
         from fenice_amd.fvg import config_from_keys
         ora = Oracle()
-        cfg = config_from_keys(run_keys(n, "fp64", quadrature))
+        cfg = config_from_keys(keys)
         U = U0.copy()
+        lag = [0.0]
 
         def step(t):
-            return ora.step(cfg, U, t, 1e9)[0]
+            dt, _, lag[0], _ = ora.step(cfg, U, t, 1e9, 0.0, lag[0])
+            return dt
         kind = "port"
     t = 0.0
     t0 = time.perf_counter()
     t += step(t)   # first step: first-touch of every work array, not timed
     warm = time.perf_counter() - t0
-    steps = max(1, min(3, int(budget_s / max(warm, 1e-3))))
+    steps = max(1, min(cap, int(budget_s / max(warm, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(steps):
         t += step(t)
     el = time.perf_counter() - t0
-    return {"value": n ** 3 * 3 * steps / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": kind,
-            "sample": f"3-D Taylor-Green {n}^3 WENO5-Z+HLLC+SSP-RK3 NS" + (", fourth-order flux quadrature" if quadrature == "fourth" else "") +
-                      f", {steps} timed step(s) after 1 untimed warm-up step "
+    return {"value": cells * nst * steps / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": kind,
+            "sample": desc + f", {steps} timed step(s) after 1 untimed warm-up step "
                       f"({el:.1f} s timed), OpenMP {threads} threads" + (", AVX2 kernels" if kind == "reference" else "")}
 
 
-def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint"):
+def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv"):
     env = dict(os.environ)
     env.pop("FENICE_NO_SIMD", None)
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-budget", str(budget_s),
-                        "--quadrature", quadrature], capture_output=True, text=True, env=env)
+                        "--quadrature", quadrature, "--workload", workload], capture_output=True, text=True, env=env)
     if r.returncode != 0:
         return {"value": None, "unit": "Mcell-updates/s per RK stage", "cores": None, "kind": None,
                 "sample": "failed: " + (r.stderr or r.stdout)[-300:]}
@@ -199,7 +216,8 @@ def side_workload(name: str) -> dict:
     """Throughput of the other configurations of the path (parity cases, not the headline): the 1-D ensemble
     (1024 Riemann problems of 128..8192 cells to t = 0.15, one launch after the autotuner), the isentropic vortex at
     256^2 under MUSCL-Hancock (CFL 0.7, 200 steps, graph replay), and the Kelvin-Helmholtz layer at 2048^2
-    (WENO5-Z + SSP-RK3, 30 steps), and the 512^3 Taylor-Green vortex in its three precision modes."""
+    (WENO5-Z + SSP-RK3, 30 steps), the 512^3 Taylor-Green vortex in its three precision modes (with the kinetic energy,
+    its dissipation and the density error against FP64 every 100 steps), and the fourth-order flux quadrature."""
     from fenice_amd import fvg
     from fenice_amd.workloads import ensemble_keys, kelvin_helmholtz, kh_keys, riemann_ensemble
     if name == "ensemble":
@@ -224,7 +242,7 @@ def side_workload(name: str) -> dict:
             el = time.perf_counter() - t0
         return {"workload": "isentropic vortex 256^2, MUSCL Van Leer + HLLC, MUSCL-Hancock, CFL 0.7, 200 steps "
                             "(graph replay)", "value": round(65536 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
-                "ms_per_step": round(el / k * 1e3, 4), "steps": k}
+                "ms_per_step": round(el / k * 1e3, 4), "steps": k, "cpu_baseline": cpu_baseline(256, 10.0, workload="vortex")}
     if name == "kh":
         n = 2048
         with fvg.Solver(fvg.config_from_keys(kh_keys(n))) as s:
@@ -235,15 +253,18 @@ def side_workload(name: str) -> dict:
             el = time.perf_counter() - t0
         return {"workload": "Kelvin-Helmholtz 2048^2, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic, 30 steps",
                 "value": round(n * n * 3 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
-                "ms_per_step": round(el / k * 1e3, 4), "steps": k}
+                "ms_per_step": round(el / k * 1e3, 4), "steps": k, "cpu_baseline": cpu_baseline(n, 15.0, workload="kh")}
     if name == "tgv512":
         # the 512^3 Taylor-Green vortex with FP32 fluxes and the learned indicator beside the plain FP64 path: each mode
-        # is timed on 8 steps after 1 warm-up step, then restarted from the initial field for 10 steps at a fixed dt
-        # (below the CFL step), whose density is compared with the FP64 run's (L1 and Linf, relative)
+        # is timed on 8 steps after 1 warm-up step, then restarted from the initial field for 300 steps at a fixed dt
+        # (3e-4, below the CFL step of 3.6e-4); every 100 steps the kinetic energy (volume average) and, against the FP64 run, the
+        # density's L1 and Linf errors (relative) are recorded, and the kinetic-energy dissipation rate -dE_k/dt over
+        # the last 100 steps
         n = 512
         U0 = tgv_state(n)
         cells = n ** 3
-        modes, rho64 = {}, None
+        dt, marks, per = 3e-4, 3, 100
+        modes, rho64 = {}, []
         for label, flux, ind in (("fp64", "fp64", "none"), ("fp32", "fp32", "none"), ("fp32+learned", "fp32", "learned")):
             with make_solver(n, flux, ind) as s:
                 s.upload(U0)
@@ -251,21 +272,33 @@ def side_workload(name: str) -> dict:
                 t0 = time.perf_counter()
                 _, k, _ = s.advance(0.0, 1e9, max_steps=8)
                 el = time.perf_counter() - t0
+                m = {"value": round(cells * 3 * k / el / 1e6, 1), "ms_per_step": round(el / k * 1e3, 3), "steps": k, "history": []}
+                print(f"tgv512 {label}: {m['value']} Mcell-updates/s per stage", file=sys.stderr, flush=True)
                 s.upload(U0)
                 t = 0.0
-                for _ in range(10):
-                    t += s.step(t, 1e9, 4e-4).dt
-                rho = s.download()[:, 0].copy()
-                m = {"value": round(cells * 3 * k / el / 1e6, 1), "ms_per_step": round(el / k * 1e3, 3), "steps": k}
+                ke_prev = float(s.conserved_totals()[1])
+                for mark in range(1, marks + 1):
+                    for _ in range(per):
+                        t += s.step(t, 1e9, dt).dt
+                    ke = float(s.conserved_totals()[1])
+                    rho = s.download()[:, 0].copy()
+                    h = {"step": mark * per, "t": round(t, 6), "kinetic_energy": ke, "dissipation": (ke_prev - ke) / (per * dt)}
+                    ke_prev = ke
+                    if label == "fp64":
+                        rho64.append(rho)
+                    else:
+                        d = np.abs(rho - rho64[mark - 1])
+                        h["l1_rel"] = float(d.sum() / np.abs(rho64[mark - 1]).sum())
+                        h["linf_rel"] = float(d.max() / np.abs(rho64[mark - 1]).max())
+                    del rho
+                    m["history"].append(h)
+                    print(f"tgv512 {label}: step {mark * per} {h}", file=sys.stderr, flush=True)
                 if ind == "learned":
                     _, total, off, audits = s.indicator_stats()
                     m["indicator"] = {"audits": audits, "switched_off": off, "tp_fp_fn_tn_all_audits": total}
-            if rho64 is None:
-                rho64 = rho
-            else:
-                d = np.abs(rho - rho64)
-                m["error_vs_fp64"] = {"l1_rel": float(d.sum() / np.abs(rho64).sum()), "linf_rel": float(d.max() / np.abs(rho64).max()),
-                                      "steps": 10, "dt": 4e-4}
+            if label != "fp64":
+                last = m["history"][-1]
+                m["error_vs_fp64"] = {"l1_rel": last["l1_rel"], "linf_rel": last["linf_rel"], "steps": marks * per, "dt": dt}
             modes[label] = m
         base = modes["fp64"]["value"]
         for m in modes.values():
@@ -325,11 +358,12 @@ def main():
     ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
     ap.add_argument("--quadrature", default="midpoint", choices=["midpoint", "fourth"], help="flux quadrature of the CPU worker")
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "vortex", "kh"], help="workload of the CPU worker")
     ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512", "fourth"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
     if a.cpu_worker:
-        print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature)))
+        print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature, a.workload)))
         return
     if a.side:
         print(json.dumps(side_workload(a.side)), flush=True)
