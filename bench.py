@@ -144,9 +144,70 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", w
                       f"({el:.1f} s timed), OpenMP {threads} threads" + (", AVX2 kernels" if kind == "reference" else "")}
 
 
+_ENS = {}
+
+
+def _ens_init():
+    """A worker of the ensemble sample: the reference's library and the ensemble's instances."""
+    from fenice_amd.workloads import ensemble_keys, riemann_ensemble
+    L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libfenice_ref.so"))
+    L.ref_create.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]
+    L.ref_create.restype = C.c_void_p
+    L.ref_set_state.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.ref_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.ref_destroy.argtypes = [C.c_void_p]
+    import tempfile
+    case_dir = tempfile.mkdtemp(prefix="fvg_case_")
+    case_file = os.path.join(case_dir, "case.toml")
+    with open(case_file, "w") as f:
+        f.write('[case]\nname = "Uniform run"\n\n[initial]\nstate = { rho = 1.0, u = 0.0, v = 0.0, p = 1.0 }\n')
+    n, U, _ = riemann_ensemble()
+    _ENS.update(L=L, case=case_file.encode(), keys=ensemble_keys(), n=n, U=U, off=np.concatenate([[0], np.cumsum(n)]))
+
+
+def _ens_run(args):
+    """Instance i for `steps` steps (or to t_end) in this process: its cell-stage updates."""
+    i, steps = args
+    L, keys = _ENS["L"], dict(_ENS["keys"], nx=str(int(_ENS["n"][i])))
+    ks = [k.encode() for k in keys]
+    vs = [str(v).encode() for v in keys.values()]
+    h = L.ref_create(_ENS["case"], len(ks), (C.c_char_p * len(ks))(*ks), (C.c_char_p * len(vs))(*vs))
+    if not h:
+        return 0
+    U = np.zeros((int(_ENS["n"][i]), 5))
+    U[:, :4] = _ENS["U"][_ENS["off"][i]:_ENS["off"][i + 1]]
+    L.ref_set_state(h, U.ctypes.data_as(C.POINTER(C.c_double)))
+    dt, sm = C.c_double(), C.c_double()
+    t, k = 0.0, 0
+    while k < steps and t < 0.15:
+        L.ref_step(h, t, 0.15, 0.0, C.byref(dt), C.byref(sm))
+        t += dt.value
+        k += 1
+    L.ref_destroy(h)
+    return int(_ENS["n"][i]) * 2 * k
+
+
+def cpu_ensemble_worker(procs: int, steps: int) -> dict:
+    """The reference on the ensemble, process-parallel: every instance for the first `steps` steps in one of `procs`
+    single-threaded processes."""
+    import multiprocessing as mp
+    from fenice_amd.workloads import riemann_ensemble
+    ninst = len(riemann_ensemble()[0])
+    with mp.get_context("fork").Pool(procs, initializer=_ens_init) as pool:
+        pool.map(_ens_run, [(i, 1) for i in range(procs)])   # warm every process (library, instances)
+        t0 = time.perf_counter()
+        done = sum(pool.map(_ens_run, [(i, steps) for i in range(ninst)], chunksize=1))
+        el = time.perf_counter() - t0
+    return {"value": done / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": procs, "kind": "reference",
+            "sample": f"the 1024 instances, each for its first {steps} steps (SSP-RK2), in {procs} single-threaded processes "
+                      f"({el:.1f} s)"}
+
+
 def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv"):
     env = dict(os.environ)
     env.pop("FENICE_NO_SIMD", None)
+    if workload == "ensemble":
+        env["OMP_NUM_THREADS"] = "1"
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-budget", str(budget_s),
                         "--quadrature", quadrature, "--workload", workload], capture_output=True, text=True, env=env)
     if r.returncode != 0:
@@ -231,7 +292,8 @@ def side_workload(name: str) -> dict:
                             "to t = 0.15", "block": block, "kernel_ms": round(r.ms, 2),
                 "value": round(stages / (r.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                 "instances_done": int((r.t >= 0.15).sum()), "diverged": int(r.diverged.sum()),
-                "steps_max": int(r.steps.max()), "steps_total": int(r.steps.sum())}
+                "steps_max": int(r.steps.max()), "steps_total": int(r.steps.sum()),
+                "cpu_baseline": cpu_baseline(16, 20, workload="ensemble")}
     if name == "vortex":
         from fenice_amd.workloads import vortex_keys, vortex_state
         with fvg.Solver(fvg.config_from_keys(vortex_keys(256, cfl="0.7"))) as s:
@@ -358,12 +420,15 @@ def main():
     ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
     ap.add_argument("--quadrature", default="midpoint", choices=["midpoint", "fourth"], help="flux quadrature of the CPU worker")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "vortex", "kh"], help="workload of the CPU worker")
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "vortex", "kh", "ensemble"], help="workload of the CPU worker")
     ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512", "fourth"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
     if a.cpu_worker:
-        print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature, a.workload)))
+        if a.workload == "ensemble":   # --n: processes, --cpu-budget: steps per instance
+            print(json.dumps(cpu_ensemble_worker(a.n, int(a.cpu_budget))))
+        else:
+            print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature, a.workload)))
         return
     if a.side:
         print(json.dumps(side_workload(a.side)), flush=True)
