@@ -66,7 +66,7 @@ __device__ __forceinline__ void q4_load(const double* planes, long long n, long 
 
 // R: the reconstruction, 2 WENO3-Z (four cells t-2 .. t+1), 3 WENO5-Z (six cells t-3 .. t+2)
 template <int NV, int D, int R>
-__global__ __launch_bounds__(256) void k4_states(KP p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k4_states(KP p) {
     const long long f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= p.nfa[D] || p.st->done) return;
     int i, j, k, t;
