@@ -69,6 +69,23 @@ def test_fp32_shock_2d_stays_close():
     assert l1 < 1e-5, l1
 
 
+@pytest.mark.parametrize("dims", [(48, 40, 1), (20, 18, 16)])
+def test_fp32_inflow_and_nonreflecting_sides_stay_close(dims):
+    # a mass-flow inlet and a non-reflecting outlet (their side states in FP64, read by the FP32 faces), with
+    # boundary_order 2 ghost slabs: 10 steps at a fixed dt stay at FP32 distance from the FP64 run
+    say(f"fp32 inflow / non-reflecting {dims}")
+    io = dict(bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_ymin="wall", bc_ymax="no_slip", u_in="0.5", v_in="0.1",
+              rho_in="1.2", p_out="0.9", boundary_order="2")
+    if dims[2] > 1:
+        io.update(bc_zmin="periodic", bc_zmax="periodic")
+    keys = base_keys(*dims, periodic=False, reconstruction="weno5", **io)
+    U = smooth_state(*dims, 9, mach=0.3)
+    U64, _ = run(keys, U, 10, 1e-3)
+    U32, _ = run(dict(keys, flux_precision="fp32"), U, 10, 1e-3)
+    l1 = np.abs(U32[:, 0] - U64[:, 0]).sum() / np.abs(U64[:, 0]).sum()
+    assert l1 < 1e-6, l1
+
+
 def test_fp32_rejects_hancock_predictor():
     say("hancock rejected")
     keys = base_keys(16, 12, reconstruction="muscl", time_scheme="SSP Runge-Kutta 2", flux_precision="fp32")
