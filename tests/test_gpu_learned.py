@@ -116,16 +116,25 @@ def learned_keys(nx, ny, nz, **kw):
     return base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32", indicator="learned", **kw)
 
 
-@pytest.mark.parametrize("dims", [(20, 18, 1), (14, 12, 10)])
-def test_all_troubled_equals_fp32_weno(dims):
-    # a network that calls every face troubled reproduces the FP32 WENO5-Z path bit for bit
-    say("all troubled", dims)
+INFLOW = dict(bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_ymin="wall", bc_ymax="no_slip", u_in="0.5", v_in="0.1",
+              rho_in="1.2", p_out="0.9", boundary_order="2")
+
+
+@pytest.mark.parametrize("dims,sides", [((20, 18, 1), "periodic"), ((14, 12, 10), "periodic"), ((20, 18, 1), "inflow"),
+                                        ((14, 12, 10), "inflow")])
+def test_all_troubled_equals_fp32_weno(dims, sides):
+    # a network that calls every face troubled reproduces the FP32 WENO5-Z path bit for bit, also with a mass-flow
+    # inlet and a non-reflecting outlet
+    say("all troubled", dims, sides)
     nx, ny, nz = dims
     U = shock_state(nx, ny, nz, 9)
+    extra = {}
+    if sides == "inflow":
+        extra = dict(INFLOW, **(dict(bc_zmin="periodic", bc_zmax="periodic") if nz > 1 else {}))
+    base = dict(base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32"), **extra)
+    lkeys = dict(learned_keys(nx, ny, nz), **extra)
     outs = []
-    for keys, params in ((base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32"), None),
-                         (learned_keys(nx, ny, nz), const_params(30.0)),
-                         (learned_keys(nx, ny, nz), None)):   # untrained: every face troubled
+    for keys, params in ((base, None), (lkeys, const_params(30.0)), (lkeys, None)):   # untrained: every face troubled
         with fvg.Solver(fvg.config_from_keys(keys)) as s:
             if params is not None:
                 s.set_indicator(params, 0.0)
