@@ -181,7 +181,7 @@ KP make_kp(fvg_ctx* c, const double* Uin) {
 // face passes read
 int enqueue_massflow(fvg_ctx* c, const KP& p) {
     int rc = FVG_OK;
-    if (c->massflow && (rc = launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_massflow, dim3(6), dim3(kBlock), 0, c->stream, p); })))
+    if (c->massflow && (rc = launch(c, FAM_CONTROL, [&] { hipLaunchKernelGGL(k_massflow, dim3(6), dim3(1024), 0, c->stream, p); })))
         return rc;
     for (int s = 0; s < 6 && !rc; ++s) {
         if (!(c->nrbc_sides >> s & 1)) continue;

@@ -43,19 +43,22 @@ template __global__ void k_nrbc<5, 4>(KP);
 template __global__ void k_nrbc<5, 5>(KP);
 
 // The mass-flow inlet velocity of every mass-flow side (apply_side, Boundaries.cpp:285-294): rho_in u_in over the
-// mean density of the side's boundary cells, floored at 1e-5. One block per side; the densities are summed in the
-// reference's boundary order (Core.hpp:121-151) by one thread, a chunk of the block's loads at a time, so that the
-// sum has the reference's bits. The result goes to DevState::ut, which the ghost states of the residual read.
-__global__ __launch_bounds__(256) void k_massflow(KP p) {
-    __shared__ double rho[256];
+// mean density of the side's boundary cells, floored at 1e-5. One block of kMassflowThreads per side; the densities
+// are summed in the reference's boundary order (Core.hpp:121-151) by one thread, a chunk of kMassflowChunk values
+// loaded into LDS by the whole block at a time, so that the sum has the reference's bits. The result goes to
+// DevState::ut, which the ghost states of the residual read.
+constexpr int kMassflowThreads = 1024, kMassflowChunk = 4096;
+__global__ __launch_bounds__(kMassflowThreads) void k_massflow(KP p) {
+    __shared__ double rho[kMassflowChunk];
     const int side = blockIdx.x;
     if (bc_of(p.bp, side) != 4 || (side >= 4 && !p.is3d) || p.st->done) return;   // uniform over the block
     const long long nx = p.nx, ny = p.ny, nz = p.nz;
     const long long n = side < 2 ? ny * nz : (side < 4 ? nx * nz : nx * ny);
     double sum = 0.0;
-    for (long long base = 0; base < n; base += blockDim.x) {
-        const long long k = base + threadIdx.x;
-        if (k < n) {
+    for (long long base = 0; base < n; base += kMassflowChunk) {
+        for (int l = threadIdx.x; l < kMassflowChunk; l += blockDim.x) {
+            const long long k = base + l;
+            if (k >= n) break;
             long long c;
             switch (side) {
             case 0: c = k * nx; break;
@@ -65,12 +68,13 @@ __global__ __launch_bounds__(256) void k_massflow(KP p) {
             case 4: c = k; break;
             default: c = (nz - 1) * ny * nx + k; break;
             }
-            rho[threadIdx.x] = smax(p.U[c], kEosEps);   // the density of cons_to_prim
+            rho[l] = smax(p.U[c], kEosEps);   // the density of cons_to_prim
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            const long long m = n - base < (long long)blockDim.x ? n - base : (long long)blockDim.x;
-            for (long long t = 0; t < m; ++t) sum += rho[t];
+            const int m = n - base < kMassflowChunk ? (int)(n - base) : kMassflowChunk;
+#pragma unroll 16
+            for (int l = 0; l < m; ++l) sum += rho[l];   // in order: the loads run ahead of the chain of adds
         }
         __syncthreads();
     }
