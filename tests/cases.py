@@ -132,6 +132,19 @@ def matrix():
     out.append(("3d-inflow-nrbc-z-walls", base_keys(8, 9, 7, periodic=False, boundary_order=1, **dict(io, bc_zmin="wall",
                                                                                                        bc_zmax="nonreflecting")), "smooth"))
     out.append(("3d-inflow-nrbc-hancock", base_keys(8, 9, 7, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **io3), "shock"))
+    # a velocity inlet and a pressure outlet (velocity kept where the flow leaves, zeroed where it enters; pressure
+    # p_out), with a transmissive side and a moving wall, in 2-D and 3-D
+    vo = dict(bc_xmin="vel_inlet", bc_xmax="outlet", bc_ymin="transmissive", bc_ymax="moving_wall", u_in="0.6", v_in="0.1",
+              p_out="0.8")
+    vo3 = dict(vo, bc_zmin="outlet", bc_zmax="vel_inlet")
+    for bo in (1, 2):
+        out.append((f"2d-velin-outlet-muscl-bo{bo}", base_keys(12, 10, periodic=False, boundary_order=bo, **vo), "shock"))
+    out.append(("2d-velin-outlet-weno5", base_keys(12, 10, periodic=False, boundary_order=2, reconstruction="weno5", **vo), "shock"))
+    out.append(("2d-velin-outlet-hancock", base_keys(12, 10, periodic=False, boundary_order=2, time_scheme="MUSCL-Hancock", **vo), "shock"))
+    out.append(("2d-velin-outlet-visc", base_keys(12, 10, periodic=False, boundary_order=2, use_viscous="true", mu="0.01", **vo), "smooth"))
+    out.append(("3d-velin-outlet-weno5-visc", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno5", use_viscous="true",
+                                                        mu="0.01", **vo3), "smooth"))
+    out.append(("3d-velin-outlet-hancock-bo1", base_keys(8, 9, 7, periodic=False, boundary_order=1, time_scheme="MUSCL-Hancock", **vo3), "shock"))
     return out
 
 
