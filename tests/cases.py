@@ -174,6 +174,12 @@ def fourth_matrix():
                                                   **dict(walls, bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_zmin="periodic",
                                                          bc_zmax="periodic", u_in="0.5", v_in="0.1", rho_in="1.2", p_out="0.9"), **F),
          "shock"),
+        ("2d-velin-outlet-weno5-fourth-bo1", base_keys(12, 10, periodic=False, boundary_order=1, reconstruction="weno5",
+                                                       **dict(walls, bc_xmin="vel_inlet", bc_xmax="outlet", u_in="0.6", v_in="0.1",
+                                                              p_out="0.8"), **F), "shock"),
+        ("3d-velin-outlet-weno3-fourth-visc", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno3", use_viscous="true",
+                                                        mu="0.01", **dict(walls3, bc_xmin="vel_inlet", bc_xmax="outlet", bc_zmin="outlet",
+                                                                          u_in="0.6", v_in="0.1", p_out="0.8"), **F), "smooth"),
     ]
 
 
