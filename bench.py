@@ -29,8 +29,15 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, published (FMA counted as 
 BYTES_PER_CELL_STAGE_RK3_3D = 8.0 * 5 * (2 + 3 + 3) / 3.0   # 106.7 B: stage input, U^n on stages 2-3, output
 # HBM bytes of one stage of the 256^3 workload (k_cells + k_faces x/y/z + k_update, the update averaged over the
 # three SSP-RK3 stages), from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB
-# device copy reads in the same runs); the runs and per-kernel figures are in profiles/tgv256_weno5_rk3_<flux>.md
-MEASURED_STAGE_TRAFFIC = {("fp64", 256): 16.776e9, ("fp32", 256): 7.678e9}
+# device copy reads in the same runs); the runs and per-kernel figures are in the profile files named here. These
+# are figures of those counter runs, not of the run that prints them.
+MEASURED_STAGE_TRAFFIC = {("fp64", 256): (16.776e9, "profiles/tgv256_weno5_rk3_fp64.md"),
+                          ("fp32", 256): (7.678e9, "profiles/tgv256_weno5_rk3_fp32.md")}
+# VALU instructions per wavefront (SQ_INSTS_VALU / SQ_WAVES, one face or cell per lane) of each kernel of the
+# workload, from the same counter runs; with this run's event time of the kernel they give its VALU issue rate. A
+# wave64 VALU instruction occupies a SIMD for 4 cycles; MI355X has 1,024 SIMDs at up to 2.4 GHz.
+VALU_PER_WAVE = {"fp64": ({"k_cells": 552, "k_faces_x": 2137, "k_faces_y": 2141, "k_faces_z": 2110}, "profiles/tgv256_weno5_rk3_fp64.md")}
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
 from fenice_amd.workloads import taylor_green_keys as run_keys  # noqa: E402
@@ -84,11 +91,23 @@ def cpu_workload(name: str, n: int, quadrature: str = "midpoint"):
                 f"isentropic vortex {n}^2 MUSCL Van Leer+HLLC+MUSCL-Hancock, CFL 0.7")
     if name == "kh":
         return kh_keys(n), five(kelvin_helmholtz(n)), n * n, 3, 3, f"Kelvin-Helmholtz {n}^2 WENO5-Z+HLLC+SSP-RK3"
-    return (run_keys(n, "fp64", quadrature), tgv_state(n), n ** 3, 3, 3,
+    return (run_keys(n, "fp64", quadrature), tgv_state(n), n ** 3, 3, 12,
             f"3-D Taylor-Green {n}^3 WENO5-Z+HLLC+SSP-RK3 NS" + (", fourth-order flux quadrature" if quadrature == "fourth" else ""))
 
 
-def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv") -> dict:
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or '' where it is not given."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv", dump: str = "") -> dict:
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libfenice_ref.so")
     keys, U0, cells, nst, cap, desc = cpu_workload(workload, n, quadrature)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
@@ -111,11 +130,17 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", w
             L.ref_last_error.restype = C.c_char_p
             raise RuntimeError(L.ref_last_error().decode())
         L.ref_set_state(h, U0.ctypes.data_as(C.POINTER(C.c_double)))
+        L.ref_get_state.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         dt, s = C.c_double(), C.c_double()
 
         def step(t):
             L.ref_step(h, t, 1e9, 0.0, C.byref(dt), C.byref(s))
             return dt.value
+
+        def state():
+            out = np.empty_like(U0)
+            L.ref_get_state(h, out.ctypes.data_as(C.POINTER(C.c_double)))
+            return out
         kind = "reference"
     else:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -129,6 +154,9 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", w
         def step(t):
             dt, _, lag[0], _ = ora.step(cfg, U, t, 1e9, 0.0, lag[0])
             return dt
+
+        def state():
+            return U
         kind = "port"
     t = 0.0
     t0 = time.perf_counter()
@@ -139,9 +167,14 @@ def cpu_baseline_worker(n: int, budget_s: float, quadrature: str = "midpoint", w
     for _ in range(steps):
         t += step(t)
     el = time.perf_counter() - t0
-    return {"value": cells * nst * steps / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": kind,
-            "sample": desc + f", {steps} timed step(s) after 1 untimed warm-up step "
-                      f"({el:.1f} s timed), OpenMP {threads} threads" + (", AVX2 kernels" if kind == "reference" else "")}
+    out = {"value": cells * nst * steps / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": kind,
+           "cpu_model": cpu_model(),
+           "sample": desc + f", {steps} timed step(s) after 1 untimed warm-up step "
+                     f"({el:.1f} s timed), OpenMP {threads} threads" + (", AVX2 kernels" if kind == "reference" else "")}
+    if dump:   # the state after every step taken (warm-up and timed), for the device's comparison
+        np.save(dump, state())
+        out["state_steps"], out["state_t"] = steps + 1, t
+    return out
 
 
 _ENS = {}
@@ -199,17 +232,18 @@ def cpu_ensemble_worker(procs: int, steps: int) -> dict:
         done = sum(pool.map(_ens_run, [(i, steps) for i in range(ninst)], chunksize=1))
         el = time.perf_counter() - t0
     return {"value": done / el / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": procs, "kind": "reference",
-            "sample": f"the 1024 instances, each for its first {steps} steps (SSP-RK2), in {procs} single-threaded processes "
+            "cpu_model": cpu_model(), "sample": f"the 1024 instances, each for its first {steps} steps (SSP-RK2), in {procs} single-threaded processes "
                       f"({el:.1f} s)"}
 
 
-def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv"):
+def cpu_baseline(n: int, budget_s: float, quadrature: str = "midpoint", workload: str = "tgv", dump: str = ""):
     env = dict(os.environ)
     env.pop("FENICE_NO_SIMD", None)
     if workload == "ensemble":
         env["OMP_NUM_THREADS"] = "1"
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-budget", str(budget_s),
-                        "--quadrature", quadrature, "--workload", workload], capture_output=True, text=True, env=env)
+                        "--quadrature", quadrature, "--workload", workload] + (["--dump", dump] if dump else []),
+                       capture_output=True, text=True, env=env)
     if r.returncode != 0:
         return {"value": None, "unit": "Mcell-updates/s per RK stage", "cores": None, "kind": None,
                 "sample": "failed: " + (r.stderr or r.stdout)[-300:]}
@@ -253,6 +287,23 @@ def measure(U0, n, flux, indicator, weights, steps, warmup):
                          "missed_troubled_fraction": total[2] / max(1, total[0] + total[2]),
                          "smooth_fraction_of_faces": (total[2] + total[3]) / max(1, sum(total))}
     return el, k, times, tot0, tot1, ind_stats
+
+
+def compare_with_reference(U0, n, flux, indicator, weights, path, steps) -> dict:
+    """The device's state after `steps` steps of the workload from U0 against the reference's (path, .npy)."""
+    with make_solver(n, flux, indicator, weights) as s:
+        s.upload(U0)
+        t = 0.0
+        for _ in range(steps):
+            t += s.step(t, 1e9).dt
+        Ud = s.download()
+    Ur = np.load(path)
+    same = bool(np.array_equal(Ud.view(np.uint64), Ur.view(np.uint64)))
+    d = np.abs(Ud[:, 0] - Ur[:, 0])
+    out = {"mesh": n, "steps": steps, "t": t, "bitwise_equal": same, "l1_rel": float(d.sum() / np.abs(Ur[:, 0]).sum()),
+           "linf_rel": float(d.max() / np.abs(Ur[:, 0]).max())}
+    del Ud, Ur
+    return out
 
 
 def other_modes(U0, n, steps, weights):
@@ -421,6 +472,8 @@ def main():
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
     ap.add_argument("--quadrature", default="midpoint", choices=["midpoint", "fourth"], help="flux quadrature of the CPU worker")
     ap.add_argument("--workload", default="tgv", choices=["tgv", "vortex", "kh", "ensemble"], help="workload of the CPU worker")
+    ap.add_argument("--dump", default="", help="CPU worker: write its final state here (.npy)")
+    ap.add_argument("--no-side", action="store_true", help="skip the vortex and Kelvin-Helmholtz lines of the default run")
     ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512", "fourth"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
@@ -428,7 +481,7 @@ def main():
         if a.workload == "ensemble":   # --n: processes, --cpu-budget: steps per instance
             print(json.dumps(cpu_ensemble_worker(a.n, int(a.cpu_budget))))
         else:
-            print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature, a.workload)))
+            print(json.dumps(cpu_baseline_worker(a.n, a.cpu_budget, a.quadrature, a.workload, a.dump)))
         return
     if a.side:
         print(json.dumps(side_workload(a.side)), flush=True)
@@ -449,6 +502,16 @@ def main():
     stage_ms = sum(ms for ms, _ in fam.values()) / stages
     achieved = BYTES_PER_CELL_STAGE_RK3_3D * cells / (stage_ms * 1e-3) / 1e9
     dom = max(fam.items(), key=lambda kv: kv[1][0])
+    traffic = MEASURED_STAGE_TRAFFIC.get((a.flux, n)) if a.indicator == "none" else None
+    # the dominant kernel's VALU issue: its instructions per wave (counter run) x waves / (SIMDs x clock / 4 x time)
+    valu = None
+    vt, vsrc = VALU_PER_WAVE.get(a.flux, ({}, ""))
+    if n == 256 and a.indicator == "none" and dom[0] in vt:
+        waves = ((n + 1) * n * n + 63) // 64 if dom[0].startswith("k_faces") else cells // 64
+        avg_s = dom[1][0] / dom[1][1] * 1e-3
+        issue = vt[dom[0]] * waves * 4.0 / (SIMDS * CLOCK_HZ * avg_s)
+        valu = {"kernel": dom[0], "instr_per_wave": vt[dom[0]], "waves": waves, "avg_ms": round(avg_s * 1e3, 4),
+                "issue_frac": round(issue, 4), "instr_per_wave_source": vsrc}
     out = {
         "metric": "Mcell-updates/s per RK stage, % HBM peak, at stated L1 error vs FP64 oracle",
         "value": round(value, 3), "unit": "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
@@ -457,11 +520,14 @@ def main():
         "data": "synthetic (analytic Taylor-Green initial field)",
         "config": {"workload": f"3-D Taylor-Green vortex {n}^3, Navier-Stokes Re 1600 Pr 0.71 Mach 0.1, WENO5-Z + HLLC + SSP-RK3, "
                                "CFL 0.5, periodic", "cells": cells, "stages_per_step": 3},
-        "l1_error_vs_fp64_oracle": 0.0 if a.flux == "fp64" else None,
-        "roofline": {"bound": "hbm", "kernel": "one RK stage (k_cells, k_faces x/y/z, k_control, k_update)",
+        "l1_error_vs_fp64_oracle": None,
+        # the bound is the vector ALU (the face kernels' FP64 arithmetic); achieved / peak / frac are the HBM figures
+        # the metric asks for (algorithmic bytes of a stage over the stage's kernel time), valu the issue rate
+        "roofline": {"bound": "valu" if a.flux == "fp64" else "valu/hbm",
+                     "kernel": "one RK stage (" + ", ".join(k for k in fam if k != "k_control") + ")",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": MEASURED_STAGE_TRAFFIC.get((a.flux, n)) if a.indicator == "none" else None,
-                     "bytes_per_cell_stage": round(BYTES_PER_CELL_STAGE_RK3_3D, 2)},
+                     "traffic": traffic[0] if traffic else None, "traffic_source": traffic[1] if traffic else None,
+                     "bytes_per_cell_stage": round(BYTES_PER_CELL_STAGE_RK3_3D, 2), "valu": valu},
         "kernels": {k: {"ms_total": round(ms, 3), "launches": cnt, "avg_ms": round(ms / cnt, 4)} for k, (ms, cnt) in fam.items()},
         "dominant_kernel": dom[0],
         "conservation": {"mass_rel_drift": abs(tot1[0] - tot0[0]) / abs(tot0[0]), "energy_rel_drift": abs(tot1[2] - tot0[2]) / abs(tot0[2])},
@@ -474,7 +540,21 @@ def main():
     if a.flux == "fp64" and a.indicator == "none" and not a.no_modes:
         out["other_modes"] = other_modes(U0, n, max(10, a.steps // 3), a.weights)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_n or n, a.cpu_budget)
+        import tempfile
+        dump = os.path.join(tempfile.mkdtemp(prefix="fvg_bench_"), "ref_state.npy") if (a.cpu_n or n) == n else ""
+        cb = cpu_baseline(a.cpu_n or n, a.cpu_budget, dump=dump)
+        if dump and cb.get("state_steps") and os.path.exists(dump):
+            # the reference's state after its sample's steps from the same initial field, against the device's after
+            # as many steps: the FP64 path must give the same bits (L1 / Linf of the density, relative, otherwise)
+            out["parity_vs_reference"] = compare_with_reference(U0, n, a.flux, a.indicator, a.weights, dump, cb["state_steps"])
+            if a.flux == "fp64" and a.indicator == "none":
+                out["l1_error_vs_fp64_oracle"] = out["parity_vs_reference"]["l1_rel"]
+            os.remove(dump)
+        cb.pop("state_steps", None)
+        cb.pop("state_t", None)
+        out["cpu_baseline"] = cb
+    if a.flux == "fp64" and a.indicator == "none" and not a.no_side:
+        out["side"] = {name: side_workload(name) for name in ("vortex", "kh")}
     print(json.dumps(out), flush=True)
     if a.calibrate_copy:
         copy_calibration()

@@ -630,7 +630,11 @@ int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cach
         BCHK(b, hipMemcpyAsync(b->U, b->Usave, cb, hipMemcpyDeviceToDevice, b->stream));
         double m = 0.0;
         const int rc = run_once(b, t_probe, 0.0, max_steps, k, &m);
-        if (rc) return rc;
+        if (rc) {   // leave the ensemble as it was uploaded, as on success
+            if (hipMemcpyAsync(b->U, b->Usave, cb, hipMemcpyDeviceToDevice, b->stream) == hipSuccess)
+                (void)hipStreamSynchronize(b->stream);
+            return rc;
+        }
         if (m < best_ms) {
             best_ms = m;
             best = k;

@@ -133,9 +133,10 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
         c->use_viscous = k;
         return FVG_OK;
     }
-    if (key == "eos_type") {   // SimParams::from: "ideal gas", anything else the stiffened gas (Simulation.cpp:232-235)
-        c->eos_type = lv == "ideal gas" ? FVG_EOS_IDEAL : FVG_EOS_STIFFENED;
-        return FVG_OK;
+    if (key == "eos_type") {   // SimParams::from accepts these two labels only (Params.cpp:476-478)
+        if (lv == "ideal gas") return (c->eos_type = FVG_EOS_IDEAL), FVG_OK;
+        if (lv == "stiffened gas") return (c->eos_type = FVG_EOS_STIFFENED), FVG_OK;
+        return fail(err, n, "Unknown eos_type: '" + v + "' (use Ideal Gas or Stiffened Gas)");
     }
     if (key == "riemann_solver") {   // parse_riemann (Schemes.cpp:77-85)
         if (lv == "hllc") return (c->riemann = FVG_HLLC), FVG_OK;
@@ -262,6 +263,10 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
     if (c->boundary_order != 1 && c->boundary_order != 2) return fail(err, n, "boundary_order must be 1 or 2");
     if (!(c->nrbc_sigma >= 0.0)) return fail(err, n, "nrbc_sigma must be >= 0");   // Params.cpp:489-490
     if (!(c->nrbc_beta <= 1.0)) return fail(err, n, "nrbc_beta must be <= 1 (< 0: local Mach number)");
+    if (!(c->cfl > 0.0)) return fail(err, n, "cfl must be positive");   // Params.cpp:491-496
+    if (!(c->t_end > 0.0)) return fail(err, n, "t_end must be positive");
+    if (!(c->dt_max >= 0.0) || !std::isfinite(c->dt_max)) return fail(err, n, "dt_max must be finite and >= 0 (0: off)");
+    if (!(c->gamma > 1.0)) return fail(err, n, "gamma must be greater than 1");
     if (c->indicator == FVG_INDICATOR_LEARNED &&
         (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))
         return fail(err, n, "indicator = learned needs reconstruction = weno5, flux_precision = fp32 and a Runge-Kutta scheme");

@@ -363,16 +363,25 @@ std::map<std::string, Array> read(const std::string& path) {
         const uint32_t method = uint32_t(le(c + 10, 2)), crc = uint32_t(le(c + 16, 4));
         uint64_t csize = le(c + 20, 4), usize = le(c + 24, 4), at = le(c + 42, 4);
         const uint64_t nl = le(c + 28, 2), xl = le(c + 30, 2), cl = le(c + 32, 2);
+        // the entry's name, extra field and comment must lie inside the central directory
+        if (k + 46 + nl + xl + cl > cd_size) throw std::runtime_error("npz: bad central directory in " + path);
         const std::string name(reinterpret_cast<const char*>(c + 46), size_t(nl));
         for (uint64_t x = 46 + nl; x + 4 <= 46 + nl + xl;) {   // zip64 extra: the fields whose 32-bit value is all ones
             const uint64_t id = le(c + x, 2), sz = le(c + x + 2, 2);
+            const uint64_t end = x + 4 + sz;
+            if (end > 46 + nl + xl) throw std::runtime_error("npz: bad extra field of " + name + " in " + path);
             if (id == 0x0001) {
                 uint64_t y = x + 4;
-                if (usize == 0xFFFFFFFFu) usize = le(c + y, 8), y += 8;
-                if (csize == 0xFFFFFFFFu) csize = le(c + y, 8), y += 8;
-                if (at == 0xFFFFFFFFu) at = le(c + y, 8), y += 8;
+                auto take = [&](uint64_t& v) {
+                    if (y + 8 > end) throw std::runtime_error("npz: short zip64 field of " + name + " in " + path);
+                    v = le(c + y, 8);
+                    y += 8;
+                };
+                if (usize == 0xFFFFFFFFu) take(usize);
+                if (csize == 0xFFFFFFFFu) take(csize);
+                if (at == 0xFFFFFFFFu) take(at);
             }
-            x += 4 + sz;
+            x = end;
         }
         k += 46 + nl + xl + cl;
         if (method != 0 || csize != usize) throw std::runtime_error("npz: entry " + name + " is compressed (only stored entries are read)");

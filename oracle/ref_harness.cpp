@@ -166,6 +166,21 @@ int ref_load_checkpoint(void* hp, double* t) {
     }
 }
 
+// Simulation::volume_averages of U (n x 5) over the run's n cells: mass, kinetic energy and total energy per cell.
+int ref_volume_averages(void* hp, const double* U, double* out) {
+    try {
+        Handle* h = static_cast<Handle*>(hp);
+        fv::Field Uf(static_cast<size_t>(h->sim.mesh.num_cells()));
+        std::memcpy(Uf.data(), U, sizeof(double) * 5 * Uf.size());
+        const std::array<double, 3> v = h->sim.volume_averages(Uf, static_cast<long long>(Uf.size()));
+        for (int q = 0; q < 3; ++q) out[q] = v[q];
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 double ref_get_lagged_speed(void* hp) { return static_cast<Handle*>(hp)->sim.scheme->lagged_speed(); }
 void ref_set_lagged_speed(void* hp, double s) { static_cast<Handle*>(hp)->sim.scheme->set_lagged_speed(s); }
 

@@ -25,8 +25,9 @@ def load_golden(name):
     return dict(U0=d["U0"], U_ref=d["U_ref"], step=int(d["step"]), keys=json.loads(str(d["keys"])), case=str(d["case"]))
 
 
-def smooth_state(nx, ny, nz, seed, gamma=1.4, noise=0.02, mach=0.3):
-    """A smooth, non-trivial conservative state (n, 5) with seeded noise; rho*w = +0 in 2-D."""
+def smooth_state(nx, ny, nz, seed, gamma=1.4, noise=0.02, mach=0.3, p_inf=0.0):
+    """A smooth, non-trivial conservative state (n, 5) with seeded noise; rho*w = +0 in 2-D. p_inf > 0: the energy
+    of the stiffened gas, (p + gamma p_inf) / (gamma - 1) + kinetic (Core.hpp:209-215)."""
     rng = np.random.default_rng(seed)
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     x = (i.ravel() + 0.5) / nx * 2 * np.pi
@@ -42,15 +43,15 @@ def smooth_state(nx, ny, nz, seed, gamma=1.4, noise=0.02, mach=0.3):
     U[:, 0] = rho
     U[:, 1] = rho * u
     U[:, 2] = rho * v
-    U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    U[:, 3] = (p + gamma * p_inf) / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
     U[:, 4] = rho * w if nz > 1 else 0.0
     return U
 
 
-def shock_state(nx, ny, nz, seed, gamma=1.4):
+def shock_state(nx, ny, nz, seed, gamma=1.4, p_inf=0.0):
     """A state with strong jumps (pressure ratio 10, density ratio 8) plus seeded noise: exercises the shock blend,
     the limiters' clipping and the Hancock admissibility reverts."""
-    U = smooth_state(nx, ny, nz, seed, gamma, noise=0.01, mach=0.5)
+    U = smooth_state(nx, ny, nz, seed, gamma, noise=0.01, mach=0.5, p_inf=p_inf)
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     left = (i.ravel() + j.ravel()) < (nx + ny) // 2
     rho = np.where(left, 1.0, 0.125)
@@ -60,7 +61,7 @@ def shock_state(nx, ny, nz, seed, gamma=1.4):
     U[:, 2] *= rho
     U[:, 4] *= rho
     kin = 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 4] ** 2) / U[:, 0]
-    U[:, 3] = p / (gamma - 1.0) + kin
+    U[:, 3] = (p + gamma * p_inf) / (gamma - 1.0) + kin
     return U
 
 
@@ -145,6 +146,19 @@ def matrix():
     out.append(("3d-velin-outlet-weno5-visc", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno5", use_viscous="true",
                                                         mu="0.01", **vo3), "smooth"))
     out.append(("3d-velin-outlet-hancock-bo1", base_keys(8, 9, 7, periodic=False, boundary_order=1, time_scheme="MUSCL-Hancock", **vo3), "shock"))
+    # the stiffened gas (the reference's sod_stiffened_grav values, gamma 4.4 and p_inf 0.5): every scheme family,
+    # walls with ghost slabs, a non-reflecting outlet and the viscous term
+    st = dict(eos_type="Stiffened Gas", gamma="4.4", p_inf="0.5")
+    out.append(("2d-stiffened-muscl-walls-bo2", base_keys(12, 10, periodic=False, boundary_order=2, muscl_limiter="MC", **walls, **st),
+                "shock"))
+    out.append(("2d-stiffened-weno5-lowmach", base_keys(12, 10, reconstruction="weno5", riemann_solver="HLLC (Low-Mach Thornber)", **st),
+                "shock"))
+    out.append(("2d-stiffened-hancock", base_keys(14, 12, time_scheme="MUSCL-Hancock", **st), "shock"))
+    out.append(("2d-stiffened-inflow-nrbc", base_keys(12, 10, periodic=False, boundary_order=2, **io, **st), "smooth"))
+    out.append(("3d-stiffened-weno5-visc", base_keys(8, 9, 7, reconstruction="weno5", use_viscous="true", mu="0.01", pr="0.71", **st),
+                "smooth"))
+    out.append(("3d-stiffened-walls-weno3-bo2", base_keys(8, 9, 7, periodic=False, boundary_order=2, reconstruction="weno3", **walls3,
+                                                          **st), "shock"))
     return out
 
 
@@ -185,6 +199,9 @@ def fourth_matrix():
 
 def state_for(kind, keys, seed=7):
     nx, ny, nz = int(keys["nx"]), int(keys["ny"]), int(keys.get("nz", 1))
+    if keys.get("eos_type", "Ideal Gas").lower() == "stiffened gas":
+        gamma, p_inf = float(keys["gamma"]), float(keys["p_inf"])
+        return (shock_state if kind == "shock" else smooth_state)(nx, ny, nz, seed, gamma=gamma, p_inf=p_inf)
     return (shock_state if kind == "shock" else smooth_state)(nx, ny, nz, seed)
 
 

@@ -131,6 +131,7 @@ class Reference:
         L.ref_checkpoint_path.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.ref_save_checkpoint.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.ref_load_checkpoint.argtypes = [C.c_void_p, _D]
+        L.ref_volume_averages.argtypes = [C.c_void_p, _D, _D]
 
     def create(self, case: str, **keys) -> "RefRun":
         ks = [k.encode() for k in keys]
@@ -194,6 +195,13 @@ class RefRun:
         if self.lib.ref_step(self.h, t, t_max, dt_max, C.byref(dt), C.byref(S)) != 0:
             raise RuntimeError(self.lib.ref_last_error().decode())
         return dt.value, S.value
+
+    def volume_averages(self, U):
+        """Simulation::volume_averages of U (the reference's own sequential sums)."""
+        out = np.empty(3)
+        if self.lib.ref_volume_averages(self.h, _dp(np.ascontiguousarray(U, np.float64)), _dp(out)) != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+        return out
 
     def run(self, t_end, max_steps=0):
         t = C.c_double(0.0)

@@ -94,6 +94,15 @@ def test_scheme_combinations_checked_like_the_reference():
     (dict(nx=4, ny=4, no_such_key=1), "Unknown key"),
     (dict(nx=0, ny=4), "positive"),
     (dict(nx=4, ny=4, bc_xmin="periodic", bc_xmax="wall"), "pairs"),
+    # SimParams::from (Params.cpp:476-478, 491-496): the two EOS labels only, and the run keys' ranges
+    (dict(nx=4, ny=4, eos_type="ideal", p_inf="1"), "Unknown eos_type"),
+    (dict(nx=4, ny=4, eos_type="tait"), "Unknown eos_type"),
+    (dict(nx=4, ny=4, cfl="0"), "cfl must be positive"),
+    (dict(nx=4, ny=4, cfl="-0.5"), "cfl must be positive"),
+    (dict(nx=4, ny=4, t_end="0"), "t_end must be positive"),
+    (dict(nx=4, ny=4, dt_max="-1e-3"), "dt_max must be finite and >= 0"),
+    (dict(nx=4, ny=4, dt_max="inf"), "dt_max must be finite and >= 0"),
+    (dict(nx=4, ny=4, gamma="1.0"), "gamma must be greater than 1"),
 ])
 def test_invalid_keys_raise_value_error(keys, msg):
     with pytest.raises(ValueError, match=msg):
@@ -117,6 +126,12 @@ def test_reference_case_keys_outside_the_path():
             fvg.config_from_keys(dict(nx=8, ny=8, **{k: v}))
     with pytest.raises(ValueError, match="not a flag"):
         fvg.config_from_keys(dict(nx=8, ny=8, use_ibm="maybe"))
+
+
+def test_eos_labels():
+    from fenice_amd.config import EOS_STIFFENED
+    assert fvg.config_from_keys(dict(nx=4, ny=4, eos_type="Stiffened Gas", p_inf="0.5")).eos_type == EOS_STIFFENED
+    assert fvg.config_from_keys(dict(nx=4, ny=4, eos_type="IDEAL GAS")).eos_type == 0
 
 
 def test_dt_length():

@@ -108,6 +108,43 @@ def test_bad_checkpoints_are_refused(tmp_path):
         fvg.checkpoint_load(cfg, str(tmp_path / "missing.npz"))
 
 
+def test_malformed_central_directory_is_refused(tmp_path, monkeypatch):
+    # a central-directory entry whose name, extra field or zip64 field would reach past the directory: refused as an
+    # input error, without reading beyond it
+    cfg = fvg.config_from_keys(keys_3d())
+    U = smooth_state(6, 5, 4, 3)
+    good = str(tmp_path / "g.npz")
+    fvg.checkpoint_save(cfg, U, 0.0, 0, good)
+    raw = open(good, "rb").read()
+    last = raw.rfind(b"PK\x01\x02")   # the last entry of the central directory
+    long_name = bytearray(raw)
+    long_name[last + 28:last + 30] = (0xFFFF).to_bytes(2, "little")
+    path = str(tmp_path / "name.npz")
+    open(path, "wb").write(bytes(long_name))
+    with pytest.raises(fvg.CheckpointIOError):
+        fvg.checkpoint_load(cfg, path)
+    long_extra = bytearray(raw)
+    long_extra[last + 30:last + 32] = (0xFFFF).to_bytes(2, "little")
+    path = str(tmp_path / "extra.npz")
+    open(path, "wb").write(bytes(long_extra))
+    with pytest.raises(fvg.CheckpointIOError):
+        fvg.checkpoint_load(cfg, path)
+    # zip64 entries (forced on a small file): a zip64 extra field that claims fewer bytes than its sizes need
+    monkeypatch.setenv("FVG_NPZ_ZIP64_FROM", "1")
+    z64 = str(tmp_path / "z64.npz")
+    fvg.checkpoint_save(cfg, U, 0.0, 0, z64)
+    raw = bytearray(open(z64, "rb").read())
+    c = raw.rfind(b"PK\x01\x02")
+    nl = int.from_bytes(raw[c + 28:c + 30], "little")
+    x = c + 46 + nl
+    assert int.from_bytes(raw[x:x + 2], "little") == 0x0001
+    raw[x + 2:x + 4] = (4).to_bytes(2, "little")
+    path = str(tmp_path / "short64.npz")
+    open(path, "wb").write(bytes(raw))
+    with pytest.raises(fvg.CheckpointIOError):
+        fvg.checkpoint_load(cfg, path)
+
+
 def test_reference_reads_our_checkpoint_and_we_read_its(ref, tmp_path):
     # both directions against the reference's own writer and reader, on the MUSCL-Hancock vortex (S is the lagged
     # speed): the reference steps, our writer saves its state, a fresh reference run loads it and continues with the

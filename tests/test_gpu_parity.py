@@ -191,7 +191,7 @@ def test_vortex_hancock_cfl08_per_step_and_divergence(oracle):
     assert not oracle.step(cfg, Uo, t, 1e9, 0.0, lag)[3]
 
 
-@pytest.mark.parametrize("case", ["3d-weno5-visc", "2d-hancock", "2d-inflow-nrbc", "3d-inflow-nrbc"])
+@pytest.mark.parametrize("case", ["3d-weno5-visc", "2d-hancock", "2d-inflow-nrbc", "3d-inflow-nrbc", "3d-velin-outlet-z"])
 def test_graph_replays_match_eager_steps(oracle, case):
     # advance() replays a captured graph of four steps; checked after each of several calls (with eager work and
     # host synchronisation between the replays) against the kernel-by-kernel launches and the oracle
@@ -202,6 +202,11 @@ def test_graph_replays_match_eager_steps(oracle, case):
     elif case == "2d-hancock":
         keys = base_keys(40, 36, time_scheme="MUSCL-Hancock")
         U = smooth_state(40, 36, 1, 32, mach=0.3)
+    elif case == "3d-velin-outlet-z":
+        keys = base_keys(18, 16, 14, periodic=False, reconstruction="weno5", boundary_order="2", bc_xmin="periodic",
+                         bc_xmax="periodic", bc_ymin="wall", bc_ymax="no_slip", bc_zmin="vel_inlet", bc_zmax="outlet", u_in="0.4",
+                         v_in="0.1", p_out="0.9")
+        U = smooth_state(18, 16, 14, 35, mach=0.3)
     else:
         # the mass-flow target and the non-reflecting ghosts come from kernels of their own inside the graph
         io = dict(bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_ymin="wall", bc_ymax="no_slip", u_in="0.5", v_in="0.1",
@@ -239,7 +244,8 @@ def test_graph_replays_match_eager_steps(oracle, case):
 
 
 @pytest.mark.parametrize("case", ["weno5-visc-rk3", "muscl-rk2-walls", "fp32-weno5-rk3", "2d-weno5-visc-rk3",
-                                  "2d-muscl-rk2-walls", "fp32-learned-rk3", "2d-fp32-learned-rk3"])
+                                  "2d-muscl-rk2-walls", "fp32-learned-rk3", "2d-fp32-learned-rk3", "velin-outlet-last-axis",
+                                  "2d-inflow-nrbc-last-axis"])
 def test_fused_stages_match_separate_passes(oracle, case):
     # the last axis' faces fused with the update (stages 2-3) give the bits of the separate face and update passes,
     # and (FP64) the oracle's
@@ -254,6 +260,18 @@ def test_fused_stages_match_separate_passes(oracle, case):
                          mu="0.01", bc_xmin="wall", bc_xmax="no_slip", bc_ymin="transmissive", bc_ymax="wall",
                          bc_zmin="no_slip", bc_zmax="transmissive", boundary_order="2")
         U = shock_state(*n, 42)
+    elif case == "velin-outlet-last-axis":
+        # a velocity inlet and a pressure outlet on the last axis (z): their ghost states enter the fused faces
+        keys = base_keys(*n, periodic=False, reconstruction="weno5", use_viscous="true", mu="0.01", boundary_order="2",
+                         bc_xmin="periodic", bc_xmax="periodic", bc_ymin="wall", bc_ymax="transmissive", bc_zmin="vel_inlet",
+                         bc_zmax="outlet", u_in="0.4", v_in="0.1", p_out="0.9")
+        U = smooth_state(*n, 45, mach=0.3)
+    elif case == "inflow-nrbc-last-axis":
+        # a mass-flow inlet and a non-reflecting outlet on the last axis (y in 2-D): the side target and the stored
+        # characteristic ghosts enter the fused faces
+        keys = base_keys(*n, periodic=False, reconstruction="muscl", boundary_order="2", bc_xmin="periodic", bc_xmax="periodic",
+                         bc_ymin="massflow_inlet", bc_ymax="nonreflecting", u_in="0.3", rho_in="0.9", p_out="1.1")
+        U = smooth_state(*n, 46, mach=0.3)
     elif case == "fp32-learned-rk3":
         # the network's first-stage decisions are used by the fused last-axis faces as by the separate passes; the
         # trained network on a shock state gives both smooth and troubled faces

@@ -84,3 +84,81 @@ def test_rk_order_on_smooth_flow(oracle):
     e1 = np.abs(run(T / 4) - fine).max()
     e2 = np.abs(run(T / 8) - fine).max()
     assert 5.0 < e1 / e2 < 12.0
+
+
+def _quartic_averages(h, n, coef):
+    """Cell averages over [i h, (i+1) h] of the quartic sum_k coef[k] x^k, from its antiderivative."""
+    x = np.arange(n + 1) * h
+    Q = sum(c * x ** (k + 1) / (k + 1) for k, c in enumerate(coef))
+    return (Q[1:] - Q[:-1]) / h
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_linear_weights_are_exact_on_quartic_averages(oracle, axis):
+    # fvo_set_weno_linear (the smooth-face value of the learned indicator): the upwind-5 face value
+    # (2c0 - 13c1 + 47c2 + 27c3 - 3c4)/60 is exact on cell averages of a quartic, from either side. Density and
+    # pressure are such averages along one axis and the fluid is at rest, so the face states of an interior face are
+    # the point values, the HLLC flux of those (equal) states is (0, p, 0, 0) along the axis, and the momentum residual
+    # of an interior cell is -(P(x_{i+1/2}) - P(x_{i-1/2})) / h with P the pressure quartic. WENO5-Z (weights off the
+    # linear ones) is not exact there, which shows the check is sharp.
+    n, m, L = 32, 4, 2.0
+    h = L / n
+    prho, pp = (1.0, 0.2, -0.1, 0.03, 0.01), (1.0, 0.3, -0.2, 0.05, 0.02)
+    rho_c, p_c = _quartic_averages(h, n, prho), _quartic_averages(h, n, pp)
+    shape = (m, n) if axis == 0 else (n, m)   # (ny, nx)
+    rho = np.broadcast_to(rho_c if axis == 0 else rho_c[:, None], shape).ravel()
+    p = np.broadcast_to(p_c if axis == 0 else p_c[:, None], shape).ravel()
+    nx, ny = shape[1], shape[0]
+    keys = base_keys(nx, ny, periodic=False, reconstruction="weno5", Lx=str(L if axis == 0 else m * h),
+                     Ly=str(m * h if axis == 0 else L), bc_xmin="transmissive", bc_xmax="transmissive", bc_ymin="transmissive",
+                     bc_ymax="transmissive", shock_blend="0")
+    cfg = config_from_keys(keys)
+    U = np.zeros((nx * ny, 5))
+    U[:, 0] = rho
+    U[:, 3] = p / 0.4
+    xf = np.arange(n + 1) * h
+    P = sum(c * xf ** k for k, c in enumerate(pp))
+    want = -(P[1:] - P[:-1]) / h
+    interior = slice(3, n - 3)   # cells whose two faces have full stencils inside the mesh
+
+    def momentum(linear):
+        oracle.set_weno_linear(linear)
+        try:
+            R, _ = oracle.residual(cfg, U)
+        finally:
+            oracle.set_weno_linear(False)
+        R = R.reshape(ny, nx, 5)
+        line = R[0, :, 1] if axis == 0 else R[:, 0, 2]
+        other = R[..., 2] if axis == 0 else R[..., 1]
+        assert np.all(np.abs(other) <= 1e-12 * np.max(np.abs(want)))   # nothing moves across the axis
+        return line
+
+    scale = np.max(np.abs(want))
+    err_lin = np.max(np.abs(momentum(True)[interior] - want[interior])) / scale
+    err_weno = np.max(np.abs(momentum(False)[interior] - want[interior])) / scale
+    assert err_lin < 1e-11, err_lin
+    assert err_weno > 1e3 * max(err_lin, 1e-15), (err_weno, err_lin)
+
+
+def test_linear_weights_mirror_symmetric(oracle):
+    # the two face values of the linear-weight branch are mirror images of each other: reflecting a jumpy field
+    # x -> L - x reflects its residual, with the x momentum's sign flipped (a right value taken from the left stencil
+    # would break this)
+    nx, ny = 20, 6
+    keys = base_keys(nx, ny, periodic=False, reconstruction="weno5", bc_xmin="wall", bc_xmax="wall", bc_ymin="periodic",
+                     bc_ymax="periodic")
+    cfg = config_from_keys(keys)
+    U = shock_state(nx, ny, 1, 19)
+    M = U.reshape(ny, nx, 5)[:, ::-1].copy()
+    M[..., 1] *= -1.0
+    oracle.set_weno_linear(True)
+    try:
+        R, _ = oracle.residual(cfg, U)
+        Rm, _ = oracle.residual(cfg, M.reshape(-1, 5))
+    finally:
+        oracle.set_weno_linear(False)
+    Rm = Rm.reshape(ny, nx, 5)[:, ::-1].copy()
+    Rm[..., 1] *= -1.0
+    R = R.reshape(ny, nx, 5)
+    scale = np.max(np.abs(R), axis=(0, 1))
+    assert np.all(np.abs(Rm - R) <= 1e-12 * np.maximum(scale, 1e-30))

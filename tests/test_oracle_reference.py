@@ -76,3 +76,19 @@ def test_one_row_runs_equal_reference(oracle, ref, n, recon, bo):
         steps += 1
     assert np.array_equal(run.get_state().view(np.uint64), U.view(np.uint64))
     assert np.all(U[:, 2] == 0.0) and not np.signbit(U[:, 2]).any()
+
+
+@pytest.mark.parametrize("dims,kind", [((12, 10, 1), "shock"), ((8, 9, 7), "smooth"), ((5, 3, 3), "shock")])
+def test_volume_averages_equal_reference(oracle, ref, dims, kind):
+    # fvo_volume_averages (the checker of the device's conserved totals and monitor row) against the reference's
+    # own Simulation::volume_averages, bit for bit
+    nx, ny, nz = dims
+    from cases import base_keys
+    keys = base_keys(nx, ny, nz)
+    case = "Taylor-Green Vortex 3D" if nz > 1 else "Isentropic Vortex"
+    run = ref.create(case, **keys)
+    cfg = config_from_keys(keys)
+    U = state_for(kind, keys, seed=23)
+    a = oracle.volume_averages(cfg, U)
+    b = run.volume_averages(U)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
