@@ -58,7 +58,7 @@ SCHEMES = [
     ("walls", dict(bc_xmin="wall", bc_xmax="no_slip")),
     ("outlet", dict(bc_xmax="outlet", p_out="0.5")),
     ("periodic", dict(bc_xmin="periodic", bc_xmax="periodic", reconstruction="weno5", time_scheme="SSP Runge-Kutta 3")),
-    ("stiffened", dict(eos_type="stiffened", p_inf="0.5")),
+    ("stiffened", dict(eos_type="stiffened gas", p_inf="0.5")),
 ]
 
 
