@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libfvg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FVG_ARCH", "gfx950")
 
-SOURCES = ["fvg_kernels_f64.hip", "fvg_kernels_f32.hip", "fvg_kernels_f4.hip", "fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip",
+SOURCES = ["fvg_kernels_f64.hip", "fvg_kernels_f32.hip", "fvg_kernels_f4.hip", "fvg_kernels.hip", "fvg_api.hip", "fvg_batch.hip", "fvg_learned.hip", "fvg_noalloc.hip",
            "fvg_config.cpp", "fvg_checkpoint.cpp", "fvg_npz.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-I" + os.path.join(ROOT, "include")]

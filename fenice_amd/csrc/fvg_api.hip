@@ -46,7 +46,6 @@ constexpr int kBlock = 256;
 static_assert(kBlock >= kRowBlock, "block sizes of the kernels' launch bounds");
 constexpr int kGraphSteps = 4;   // steps per captured graph of fvg_advance
 constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along x, 16 along z
-constexpr int kTotalsBlocks = 1024;
 thread_local std::string g_create_error;
 
 enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_AUDIT,

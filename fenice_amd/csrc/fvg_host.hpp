@@ -6,6 +6,10 @@
 
 namespace fvg {
 
+// Blocks of the first pass of the conserved totals (k_totals_partial): the sums are a fixed-order tree over this
+// many per-block partial sums, so every caller that wants the same bits launches the same count.
+constexpr int kTotalsBlocks = 1024;
+
 // Mesh, numerics and viscous scalars of the kernels for a finalized configuration (buffers are left to the caller).
 void fill_kernel_params(const fvg_config& cfg, KP& p);
 

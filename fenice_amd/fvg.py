@@ -32,6 +32,7 @@ EXPORTS = [
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
     "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
     "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read", "fvg_monitor",
+    "fvg_conserved_totals_noalloc",
 ]
 
 MLP_NPARAM = 1601
@@ -134,6 +135,7 @@ def lib():
         L.fvg_checkpoint_write.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int]
         L.fvg_checkpoint_read.argtypes = [C.c_void_p, C.c_char_p, _D, _I]
         L.fvg_monitor.argtypes = [C.c_void_p, _D]
+        L.fvg_conserved_totals_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 

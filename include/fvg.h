@@ -184,6 +184,21 @@ int fvg_checkpoint_read(fvg_ctx* ctx, const char* path, double* t, int* step);
  * sequential sum to round-off, not bit for bit. */
 int fvg_conserved_totals(fvg_ctx* ctx, double out[3]);
 
+/* The volume averages of fvg_conserved_totals for a state the caller holds on the device, enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) without allocating anything: no hipMalloc, no hipFree, no context. It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). U: ncomp (4 or 5) structure-of-arrays FP64 planes of ncells values each,
+ * plane q at U + q * ncells, in the component order of a Field; out: 3 doubles of device memory (mass, kinetic
+ * energy, total energy), written when the stream reaches the call. Same kernels and block count as
+ * fvg_conserved_totals, so the same bits. Its partial sums live in one fixed device array per GPU of
+ * FVG_NOALLOC_TOTALS_SCRATCH bytes that every call on that GPU shares: calls that overlap on one GPU (other
+ * threads, other streams) race. 1 <= ncells <= FVG_NOALLOC_MAX_CELLS, else FVG_EINVAL. The first call on a device
+ * looks up that device's array (hipGetSymbolAddress) and makes the device current meanwhile; make it outside a
+ * capture. Returns without waiting for the stream. */
+#define FVG_NOALLOC_TOTALS_SCRATCH 24576
+#define FVG_NOALLOC_MAX_CELLS 2147483647LL
+int fvg_conserved_totals_noalloc(int device, const double* U, long long ncells, int ncomp, double* out, void* stream);
+
 /* The numbers of a row of the reference's monitor file (`<prefix>_monitor.csv`: step, t, mass, kinetic_energy,
  * total_energy, res_rho, res_mom, res_E; Simulation::monitor_row and its uniform-mesh caller,
  * SimulationRun.cpp:76-131, 413-424): out[0..2] the volume averages of the state (fvg_conserved_totals), out[3..5] the
