@@ -23,10 +23,10 @@ template <int NV> __global__ void k_hancock(KP);
 template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV, int WHAT> __global__ void k_totals_partial(const double*, long long, double*);
-template <int NV, int D, int USE> __global__ void k_faces_learned(KP);
+template <int NV, int D, int USE, typename T> __global__ void k_faces_learned(KP);
 template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
-template <int NV, int D> __global__ void k_audit(KP);
-template <int NV, int D> __global__ void k_samples(KP, float*, float*);
+template <int NV, int D, typename T> __global__ void k_audit(KP);
+template <int NV, int D, typename T> __global__ void k_samples(KP, float*, float*);
 template <int NV, int D, int R> __global__ void k4_states(KP);
 template <int NV, int D> __global__ void k4_points(KP);
 template <int NV, int D> __global__ void k4_fluxes(KP);
@@ -218,28 +218,26 @@ int enqueue_residual_nv(fvg_ctx* c, const double* Uin, int mode, bool audit, boo
     if ((rc = enqueue_massflow(c, p))) return rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
-    if constexpr (std::is_same<T, float>::value) {
-        if (mode == MODE_LEARNED) {
+    if (mode == MODE_LEARNED) {
 #define LFACES(KERNEL, FAM, D, GRID, BLOCK) \
-    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D>), dim3(GRID), dim3(BLOCK), 0, c->stream, p); })
-#define LEARNED(FAM, D)                                                                                                         \
-    launch(c, FAM, [&] {                                                                                                      \
-        if (reuse) hipLaunchKernelGGL((k_faces_learned<NV, D, LF_KEPT>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p); \
-        else hipLaunchKernelGGL((k_faces_learned<NV, D, LF_STORE>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p);     \
+    launch(c, FAM, [&] { hipLaunchKernelGGL((KERNEL<NV, D, T>), dim3(GRID), dim3(BLOCK), 0, c->stream, p); })
+#define LEARNED(FAM, D)                                                                                                            \
+    launch(c, FAM, [&] {                                                                                                         \
+        if (reuse) hipLaunchKernelGGL((k_faces_learned<NV, D, LF_KEPT, T>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p); \
+        else hipLaunchKernelGGL((k_faces_learned<NV, D, LF_STORE, T>), dim3(faces_grid(c, D)), dim3(faces_block(c, D)), 0, c->stream, p);     \
     })
-            if (audit) {
-                if ((rc = LFACES(k_audit, FAM_AUDIT, 0, grid_for(c->nfa[0]), kBlock)) || (rc = LFACES(k_audit, FAM_AUDIT, 1, grid_for(c->nfa[1]), kBlock)))
-                    return rc;
-                if constexpr (NV == 5)
-                    if ((rc = LFACES(k_audit, FAM_AUDIT, 2, grid_for(c->nfa[2]), kBlock))) return rc;
-            }
-            if ((rc = LEARNED(FAM_FACES_X, 0)) || (rc = LEARNED(FAM_FACES_Y, 1))) return rc;
+        if (audit) {
+            if ((rc = LFACES(k_audit, FAM_AUDIT, 0, grid_for(c->nfa[0]), kBlock)) || (rc = LFACES(k_audit, FAM_AUDIT, 1, grid_for(c->nfa[1]), kBlock)))
+                return rc;
             if constexpr (NV == 5)
-                if ((rc = LEARNED(FAM_FACES_Z, 2))) return rc;
+                if ((rc = LFACES(k_audit, FAM_AUDIT, 2, grid_for(c->nfa[2]), kBlock))) return rc;
+        }
+        if ((rc = LEARNED(FAM_FACES_X, 0)) || (rc = LEARNED(FAM_FACES_Y, 1))) return rc;
+        if constexpr (NV == 5)
+            if ((rc = LEARNED(FAM_FACES_Z, 2))) return rc;
 #undef LEARNED
 #undef LFACES
-            return FVG_OK;
-        }
+        return FVG_OK;
     }
     if constexpr (std::is_same<T, double>::value) {
         if (mode == MODE_FOURTH) {   // per axis: face states and flags, point values and their fluxes, corrected fluxes
@@ -355,12 +353,10 @@ int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const 
     for (int d = 0; d < (NV == 5 ? 2 : 1); ++d) {
         rc = launch(c, FAM_FACES_X + d, [&] {
             const dim3 g(grid_for(c->nfa[d])), b(kBlock);
-            if constexpr (std::is_same<T, float>::value) {
-                if (mode == MODE_LEARNED) {   // stages 2-3: the first stage's decisions
-                    if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0, LF_KEPT>), g, b, 0, c->stream, p);
-                    else hipLaunchKernelGGL((k_faces_learned<NV, 1, LF_KEPT>), g, b, 0, c->stream, p);
-                    return;
-                }
+            if (mode == MODE_LEARNED) {   // stages 2-3: the first stage's decisions
+                if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0, LF_KEPT, T>), g, b, 0, c->stream, p);
+                else hipLaunchKernelGGL((k_faces_learned<NV, 1, LF_KEPT, T>), g, b, 0, c->stream, p);
+                return;
             }
             if (mode == MODE_PLAIN) {
                 if (d == 0) hipLaunchKernelGGL((k_faces<NV, 0, MODE_PLAIN, T>), g, b, 0, c->stream, p);
@@ -384,11 +380,9 @@ int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const 
     case UPD_RK3_2: ZU(M, UPD_RK3_2); break; \
     default: ZU(M, UPD_RK3_3); break;        \
     }
-        if constexpr (std::is_same<T, float>::value) {
-            if (mode == MODE_LEARNED) {
-                ZU_MODE(MODE_LEARNED)
-                return;
-            }
+        if (mode == MODE_LEARNED) {
+            ZU_MODE(MODE_LEARNED)
+            return;
         }
         if (mode == MODE_PLAIN) {
             ZU_MODE(MODE_PLAIN)
@@ -1093,7 +1087,6 @@ int fvg_indicator_stats(fvg_ctx* c, unsigned long long counts[8], int* off, int*
 
 int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long nfaces) {
     if (!c || !X || !dep) return set_err(c, FVG_EINVAL, "null argument");
-    if (!c->fp32) return set_err(c, FVG_ELOGIC, "samples are taken from a context with flux_precision = fp32");
     if (axis < 0 || axis >= c->ND) return set_err(c, FVG_EINVAL, "axis out of range");
     if (nfaces != c->nfa[axis]) return set_err(c, FVG_EINVAL, "nfaces differs from the axis' face count");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1111,16 +1104,21 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
         hipFree(dd);
         return rc;
     }
-    if (c->NV == 5) {
-        hipLaunchKernelGGL((k_cells<5, float>), gc, bc, 0, c->stream, p);
-        if (axis == 0) hipLaunchKernelGGL((k_samples<5, 0>), gf, b, 0, c->stream, p, dX, dd);
-        else if (axis == 1) hipLaunchKernelGGL((k_samples<5, 1>), gf, b, 0, c->stream, p, dX, dd);
-        else hipLaunchKernelGGL((k_samples<5, 2>), gf, b, 0, c->stream, p, dX, dd);
-    } else {
-        hipLaunchKernelGGL((k_cells<4, float>), gc, bc, 0, c->stream, p);
-        if (axis == 0) hipLaunchKernelGGL((k_samples<4, 0>), gf, b, 0, c->stream, p, dX, dd);
-        else hipLaunchKernelGGL((k_samples<4, 1>), gf, b, 0, c->stream, p, dX, dd);
-    }
+    auto samples = [&](auto tag) {   // the primitives in the context's flux precision, then the axis' samples
+        using T = decltype(tag);
+        if (c->NV == 5) {
+            hipLaunchKernelGGL((k_cells<5, T>), gc, bc, 0, c->stream, p);
+            if (axis == 0) hipLaunchKernelGGL((k_samples<5, 0, T>), gf, b, 0, c->stream, p, dX, dd);
+            else if (axis == 1) hipLaunchKernelGGL((k_samples<5, 1, T>), gf, b, 0, c->stream, p, dX, dd);
+            else hipLaunchKernelGGL((k_samples<5, 2, T>), gf, b, 0, c->stream, p, dX, dd);
+        } else {
+            hipLaunchKernelGGL((k_cells<4, T>), gc, bc, 0, c->stream, p);
+            if (axis == 0) hipLaunchKernelGGL((k_samples<4, 0, T>), gf, b, 0, c->stream, p, dX, dd);
+            else hipLaunchKernelGGL((k_samples<4, 1, T>), gf, b, 0, c->stream, p, dX, dd);
+        }
+    };
+    if (c->fp32) samples(0.0f);
+    else samples(0.0);
     int rc = FVG_OK;
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, FVG_EHIP, "sample kernels failed");
     if (!rc && (hipMemcpy(X, dX, sizeof(float) * 16 * (size_t)nfaces, hipMemcpyDeviceToHost) != hipSuccess ||

@@ -267,9 +267,10 @@ int fvg_config_finalize(fvg_config* c, char* err, size_t n) {
     if (!(c->t_end > 0.0)) return fail(err, n, "t_end must be positive");
     if (!(c->dt_max >= 0.0) || !std::isfinite(c->dt_max)) return fail(err, n, "dt_max must be finite and >= 0 (0: off)");
     if (!(c->gamma > 1.0)) return fail(err, n, "gamma must be greater than 1");
-    if (c->indicator == FVG_INDICATOR_LEARNED &&
-        (c->reconstruction != FVG_RECON_WENO5 || c->flux_precision != FVG_FLUX_FP32 || c->time_scheme == FVG_TS_HANCOCK))
-        return fail(err, n, "indicator = learned needs reconstruction = weno5, flux_precision = fp32 and a Runge-Kutta scheme");
+    if (c->indicator == FVG_INDICATOR_LEARNED && (c->reconstruction != FVG_RECON_WENO5 || c->time_scheme == FVG_TS_HANCOCK))
+        return fail(err, n, "indicator = learned needs reconstruction = weno5 and a Runge-Kutta scheme");
+    if (c->indicator == FVG_INDICATOR_LEARNED && c->flux_quadrature != FVG_QUAD_MIDPOINT)
+        return fail(err, n, "indicator = learned needs flux_quadrature = midpoint");
     if (c->indicator_audit < 0 || !(c->indicator_label > 0.0) || !(c->indicator_fn_limit >= 0.0) || !(c->indicator_fn_floor >= 0.0))
         return fail(err, n, "indicator_audit must be >= 0, indicator_label > 0 and indicator_fn_limit, indicator_fn_floor >= 0");
     // a first-order scheme has no reconstruction to extend to the boundary faces (Params.cpp:460)

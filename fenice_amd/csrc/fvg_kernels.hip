@@ -284,26 +284,6 @@ __global__ __launch_bounds__(256) void k_riemann_batch(RiemannParams rs, const d
 // ------------------------------------------------------------------------------------------------------------
 template __global__ void k_hancock<4>(KP);
 template __global__ void k_hancock<5>(KP);
-template __global__ void k_faces_learned<4, 0, LF_STORE>(KP);
-template __global__ void k_faces_learned<4, 1, LF_STORE>(KP);
-template __global__ void k_faces_learned<5, 0, LF_STORE>(KP);
-template __global__ void k_faces_learned<5, 1, LF_STORE>(KP);
-template __global__ void k_faces_learned<5, 2, LF_STORE>(KP);
-template __global__ void k_faces_learned<4, 0, LF_KEPT>(KP);
-template __global__ void k_faces_learned<4, 1, LF_KEPT>(KP);
-template __global__ void k_faces_learned<5, 0, LF_KEPT>(KP);
-template __global__ void k_faces_learned<5, 1, LF_KEPT>(KP);
-template __global__ void k_faces_learned<5, 2, LF_KEPT>(KP);
-template __global__ void k_audit<4, 0>(KP);
-template __global__ void k_audit<4, 1>(KP);
-template __global__ void k_audit<5, 0>(KP);
-template __global__ void k_audit<5, 1>(KP);
-template __global__ void k_audit<5, 2>(KP);
-template __global__ void k_samples<4, 0>(KP, float*, float*);
-template __global__ void k_samples<4, 1>(KP, float*, float*);
-template __global__ void k_samples<5, 0>(KP, float*, float*);
-template __global__ void k_samples<5, 1>(KP, float*, float*);
-template __global__ void k_samples<5, 2>(KP, float*, float*);
 template __global__ void k_totals_partial<4, 0>(const double*, long long, double*);
 template __global__ void k_totals_partial<5, 0>(const double*, long long, double*);
 template __global__ void k_totals_partial<4, 1>(const double*, long long, double*);

@@ -28,5 +28,27 @@ FVG_ZUPD(5, MODE_LEARNED, float)
 FVG_ZUPD(4, MODE_PLAIN, float)
 FVG_ZUPD(4, MODE_FIRST_ORDER, float)
 FVG_ZUPD(4, MODE_LEARNED, float)
+#define FVG_LEARNED(T)                                                                \
+    template __global__ void k_faces_learned<4, 0, LF_STORE, T>(KP);                  \
+    template __global__ void k_faces_learned<4, 1, LF_STORE, T>(KP);                  \
+    template __global__ void k_faces_learned<5, 0, LF_STORE, T>(KP);                  \
+    template __global__ void k_faces_learned<5, 1, LF_STORE, T>(KP);                  \
+    template __global__ void k_faces_learned<5, 2, LF_STORE, T>(KP);                  \
+    template __global__ void k_faces_learned<4, 0, LF_KEPT, T>(KP);                   \
+    template __global__ void k_faces_learned<4, 1, LF_KEPT, T>(KP);                   \
+    template __global__ void k_faces_learned<5, 0, LF_KEPT, T>(KP);                   \
+    template __global__ void k_faces_learned<5, 1, LF_KEPT, T>(KP);                   \
+    template __global__ void k_faces_learned<5, 2, LF_KEPT, T>(KP);                   \
+    template __global__ void k_audit<4, 0, T>(KP);                                    \
+    template __global__ void k_audit<4, 1, T>(KP);                                    \
+    template __global__ void k_audit<5, 0, T>(KP);                                    \
+    template __global__ void k_audit<5, 1, T>(KP);                                    \
+    template __global__ void k_audit<5, 2, T>(KP);                                    \
+    template __global__ void k_samples<4, 0, T>(KP, float*, float*);                  \
+    template __global__ void k_samples<4, 1, T>(KP, float*, float*);                  \
+    template __global__ void k_samples<5, 0, T>(KP, float*, float*);                  \
+    template __global__ void k_samples<5, 1, T>(KP, float*, float*);                  \
+    template __global__ void k_samples<5, 2, T>(KP, float*, float*);
+FVG_LEARNED(float)
 
 }   // namespace fvg

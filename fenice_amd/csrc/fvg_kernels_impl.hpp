@@ -911,9 +911,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Learned troubled-face indicator (FP32 fluxes, WENO5-Z; fvg_learned.hpp). Faces the network calls smooth take
-// the linear upwind-5 value; the others, and every face once the indicator is off, take WENO5-Z. Faces without a
-// full stencil keep their first-order states as in k_faces.
+// Learned troubled-face indicator (WENO5-Z; fvg_learned.hpp), in either flux precision T. Faces the network calls
+// smooth take the linear upwind-5 value; the others, and every face once the indicator is off, take WENO5-Z (in T:
+// FP64 WENO5-Z with FP64 fluxes, the reference's bits). Faces without a full stencil keep their first-order states as
+// in k_faces. The network itself always runs in bfloat16 / float on the features of the float-rounded stencil.
 // ------------------------------------------------------------------------------------------------------------
 template <int NV, int D>
 __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
@@ -928,16 +929,16 @@ __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
 template <int NV, int D> __device__ __forceinline__ unsigned char* kept_of(const KP& p) {
     return D == 0 ? p.ind_kept[0] : (D == 1 ? p.ind_kept[1] : p.ind_kept[2]);
 }
-template <int NV, int D, int USE>
-__device__ __forceinline__ void learned_face(const KP& p, bool active, long long f, int i, int j, int k, int t, int n, float* F,
-                                             float& sm) {
+template <int NV, int D, int USE, typename T>
+__device__ __forceinline__ void learned_face(const KP& p, bool active, long long f, int i, int j, int k, int t, int n, T* F,
+                                             T& sm) {
     const bool upgrade = active && face_upgrade<NV, D>(p, t);
-    float c[6][NV];
+    T c[6][NV];
     bool troubled = true;
     if constexpr (USE == LF_KEPT) {
         if (upgrade) {
 #pragma unroll
-            for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
             troubled = kept_of<NV, D>(p)[f] != 0;
         }
     } else {   // LF_STORE
@@ -946,8 +947,8 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
         for (int q = 0; q < 16; ++q) x[q] = 0.0f;
         if (upgrade) {
 #pragma unroll
-            for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
-            face_features<NV>(c, float(p.weno_eps), x);
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
+            face_features<NV, T>(c, float(p.weno_eps), x);
         }
         const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
         troubled = z > p.mlp->thr;
@@ -956,11 +957,11 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
     if (!active) return;
     const bool lo = t == 0, hi = t == n;
     const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
-    float UL[NV], UR[NV];
+    T UL[NV], UR[NV];
     if (upgrade) {
         const bool smooth = !troubled && !p.ind->off;
-        float wl[NV], wr[NV];
-        const float we = float(p.weno_eps);
+        T wl[NV], wr[NV];
+        const T we = T(p.weno_eps);
         if (smooth) {
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
@@ -990,7 +991,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
     face_flux<NV>(p.rs, UL, UR, D, F, sm);
 }
 
-template <int NV, int D, int USE>
+template <int NV, int D, int USE, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_faces_learned(KP p) {
     if (p.st->done) return;   // uniform: the whole grid leaves
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
@@ -1008,19 +1009,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         face_coords<NV, D>(p, f, i, j, k, t);
     }
     const int n = ncell(p, D);
-    float F[NV], sm = 0.0f;
-    learned_face<NV, D, USE>(p, active, f, i, j, k, t, n, F, sm);
+    T F[NV], sm = T(0);
+    learned_face<NV, D, USE, T>(p, active, f, i, j, k, t, n, F, sm);
     double local = 0.0;
     if (active) {
         if (double(sm) > local) local = double(sm);
-        store_face<NV, D, float>(p, f, i, j, k, t, n, F);
+        store_face<NV, D, T>(p, f, i, j, k, t, n, F);
     }
     block_max_slot(p, local, SPD_CONV);
 }
 
 // Audit of the indicator on the stage input of an audit step: the network's decision against the label (WENO5-Z
 // weight departure above ind_label) on every face with a full stencil; counts TP, FP, FN, TN.
-template <int NV, int D>
+template <int NV, int D, typename T>
 __global__ __launch_bounds__(256) void k_audit(KP p) {
     if (p.st->done || p.ind_audit <= 0 || (p.ind->steps % p.ind_audit) != 0) return;   // uniform
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
@@ -1029,16 +1030,16 @@ __global__ __launch_bounds__(256) void k_audit(KP p) {
     int i = 0, j = 0, k = 0, t = 0;
     if (f < nf) face_coords<NV, D>(p, f, i, j, k, t);
     const bool upgrade = f < nf && face_upgrade<NV, D>(p, t);
-    float c[6][NV];
+    T c[6][NV];
     float x[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) x[q] = 0.0f;
     bool label = false;
     if (upgrade) {
 #pragma unroll
-        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
-        face_features<NV>(c, float(p.weno_eps), x);
-        label = face_departure<NV, float>(c, float(p.weno_eps)) > p.ind_label;
+        for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
+        face_features<NV, T>(c, float(p.weno_eps), x);
+        label = face_departure<NV, T>(c, T(p.weno_eps)) > T(p.ind_label);
     }
     const float z = mlp_logit_wave(p.mlp, x);
     const bool pred = z > p.mlp->thr;
@@ -1062,7 +1063,7 @@ __global__ __launch_bounds__(256) void k_audit(KP p) {
 
 // Training samples of one axis: the features (16 floats per face, the 16th = 1) and the label value of every face
 // of the current primitives; faces without a full stencil get label -1.
-template <int NV, int D>
+template <int NV, int D, typename T>
 __global__ __launch_bounds__(256) void k_samples(KP p, float* __restrict__ X, float* __restrict__ dep) {
     const int ni = p.nx + (D == 0), nj = p.ny + (D == 1), nk = p.nz + (D == 2);
     const long long nf = (long long)ni * nj * nk;
@@ -1075,11 +1076,11 @@ __global__ __launch_bounds__(256) void k_samples(KP p, float* __restrict__ X, fl
     for (int q = 0; q < 16; ++q) x[q] = 0.0f;
     float d = -1.0f;
     if (face_upgrade<NV, D>(p, t)) {
-        float c[6][NV];
+        T c[6][NV];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) stencil_prim<NV, float>(p, D, i, j, k, t - 3 + s, c[s]);
-        face_features<NV>(c, float(p.weno_eps), x);
-        d = face_departure<NV, float>(c, float(p.weno_eps));
+        for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
+        face_features<NV, T>(c, float(p.weno_eps), x);
+        d = float(face_departure<NV, T>(c, T(p.weno_eps)));
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) X[f * 16 + q] = x[q];
@@ -1189,7 +1190,7 @@ __device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j
     if constexpr (MODE == MODE_LEARNED) {
         // stages 2-3: the decisions the network made on the step's first stage, as in the separate face passes
         const long long f = D == 2 ? ((long long)k * p.ny + j) * p.nx + i : (long long)j * p.nx + i;
-        learned_face<NV, D, LF_KEPT>(p, active, f, i, j, k, kf, n, Fc, sm);
+        learned_face<NV, D, LF_KEPT, T>(p, active, f, i, j, k, kf, n, Fc, sm);
     } else {
         if (!active) return;
         T UL[NV], UR[NV];

@@ -143,8 +143,14 @@ __device__ __forceinline__ void feat5(float v0, float v1, float v2, float v3, fl
     x[4] = fmaxf(fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fmaxf(fabsf(d2), fabsf(d3))), fabsf(d4)) * r;
 }
 
-// the 15 features of a face (x[15] = 1 carries the first layer's bias)
-template <int NV> __device__ __forceinline__ void face_features(const float (*c)[NV], float eps_rel, float* x) {
+// the 15 features of a face (x[15] = 1 carries the first layer's bias) from its six-cell stencil c; an FP64
+// stencil is rounded to float first, so the network sees the same inputs in both flux precisions
+template <int NV, typename T> __device__ __forceinline__ void face_features(const T (*cs)[NV], float eps_rel, float* x) {
+    float c[6][NV];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) c[s][q] = float(cs[s][q]);
     feat5(c[0][0], c[1][0], c[2][0], c[3][0], c[4][0], c[5][0], eps_rel, x);
     feat5(c[0][3], c[1][3], c[2][3], c[3][3], c[4][3], c[5][3], eps_rel, x + 5);
     // the velocity slot with the largest total variation (the first on ties), picked by selects

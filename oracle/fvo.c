@@ -711,9 +711,9 @@ static void reconstruct_faces(const fvg_config* c, const Mesh* m, const Eos* e, 
                     } else {
                         const double c0 = cq[0], c1 = cq[stride], c2 = cq[2 * stride], c3 = cq[3 * stride], c4 = cq[4 * stride],
                                      c5 = cq[5 * stride];
-                        if (g_weno_linear) {
-                            wl[q] = (2.0 * c0 - 13.0 * c1 + 47.0 * c2 + 27.0 * c3 - 3.0 * c4) / 60.0;
-                            wr[q] = (2.0 * c5 - 13.0 * c4 + 47.0 * c3 + 27.0 * c2 - 3.0 * c1) / 60.0;
+                        if (g_weno_linear) {   /* the weighted sum times the double nearest 1/60 */
+                            wl[q] = (2.0 * c0 - 13.0 * c1 + 47.0 * c2 + 27.0 * c3 - 3.0 * c4) * (1.0 / 60.0);
+                            wr[q] = (2.0 * c5 - 13.0 * c4 + 47.0 * c3 + 27.0 * c2 - 3.0 * c1) * (1.0 / 60.0);
                         } else {
                             wl[q] = weno5z(c0, c1, c2, c3, c4, we);
                             wr[q] = weno5z(c5, c4, c3, c2, c1, we);

@@ -103,6 +103,10 @@ def test_scheme_combinations_checked_like_the_reference():
     (dict(nx=4, ny=4, dt_max="-1e-3"), "dt_max must be finite and >= 0"),
     (dict(nx=4, ny=4, dt_max="inf"), "dt_max must be finite and >= 0"),
     (dict(nx=4, ny=4, gamma="1.0"), "gamma must be greater than 1"),
+    # the learned indicator (no reference key): WENO5-Z, a Runge-Kutta scheme, the midpoint quadrature
+    (dict(nx=4, ny=4, indicator="learned", reconstruction="weno3"), "indicator = learned needs reconstruction = weno5"),
+    (dict(nx=4, ny=4, indicator="learned", reconstruction="weno5", flux_quadrature="fourth"),
+     "indicator = learned needs flux_quadrature = midpoint"),
 ])
 def test_invalid_keys_raise_value_error(keys, msg):
     with pytest.raises(ValueError, match=msg):
@@ -126,6 +130,12 @@ def test_reference_case_keys_outside_the_path():
             fvg.config_from_keys(dict(nx=8, ny=8, **{k: v}))
     with pytest.raises(ValueError, match="not a flag"):
         fvg.config_from_keys(dict(nx=8, ny=8, use_ibm="maybe"))
+
+
+def test_learned_indicator_in_both_flux_precisions():
+    for prec in ("fp64", "fp32"):
+        c = fvg.config_from_keys(dict(nx=4, ny=4, indicator="learned", reconstruction="weno5", flux_precision=prec))
+        assert c.indicator == 1 and c.flux_precision == (prec == "fp32")
 
 
 def test_eos_labels():

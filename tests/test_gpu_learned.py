@@ -77,21 +77,22 @@ def test_training_separates_a_toy_problem():
     assert loss < 0.3 * l0 and acc > 0.95, (l0, loss, acc)
 
 
-def prim_planes(U, nx, ny, nz, gamma=1.4):
+def prim_planes(U, nx, ny, nz, gamma=1.4, prec="fp32"):
     rho = U[:, 0]
     u, v, w = U[:, 1] / rho, U[:, 2] / rho, U[:, 4] / rho
     p = (gamma - 1.0) * (U[:, 3] - 0.5 * rho * (u * u + v * v + w * w))
     W = np.stack([rho, u, v, p, w]).reshape(5, nz, ny, nx)
-    return W.astype(np.float32).astype(np.float64)   # the FP32 path's primitives
+    return W.astype(np.float32).astype(np.float64) if prec == "fp32" else W   # the path's primitives
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("dims", [(24, 20, 1), (16, 14, 12)])
-def test_samples_match_cpu(dims):
-    say("samples", dims)
+def test_samples_match_cpu(dims, prec):
+    say("samples", dims, prec)
     nx, ny, nz = dims
-    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32")
+    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_precision=prec)
     U = shock_state(nx, ny, nz, 8)
-    W = prim_planes(U, nx, ny, nz)
+    W = prim_planes(U, nx, ny, nz, prec=prec)
     nv = 5 if nz > 1 else 4
     with fvg.Solver(fvg.config_from_keys(keys)) as s:
         s.upload(U)
@@ -112,27 +113,28 @@ def test_samples_match_cpu(dims):
             assert np.abs(dd[big] - dep[big]).max() < 1e-3 and np.abs(dd - dep).max() < 2e-3
 
 
-def learned_keys(nx, ny, nz, **kw):
-    return base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32", indicator="learned", **kw)
+def learned_keys(nx, ny, nz, prec="fp32", **kw):
+    return base_keys(nx, ny, nz, reconstruction="weno5", flux_precision=prec, indicator="learned", **kw)
 
 
 INFLOW = dict(bc_xmin="massflow_inlet", bc_xmax="nonreflecting", bc_ymin="wall", bc_ymax="no_slip", u_in="0.5", v_in="0.1",
               rho_in="1.2", p_out="0.9", boundary_order="2")
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("dims,sides", [((20, 18, 1), "periodic"), ((14, 12, 10), "periodic"), ((20, 18, 1), "inflow"),
                                         ((14, 12, 10), "inflow")])
-def test_all_troubled_equals_fp32_weno(dims, sides):
-    # a network that calls every face troubled reproduces the FP32 WENO5-Z path bit for bit, also with a mass-flow
-    # inlet and a non-reflecting outlet
-    say("all troubled", dims, sides)
+def test_all_troubled_equals_plain_weno(oracle, dims, sides, prec):
+    # a network that calls every face troubled reproduces the WENO5-Z path of its flux precision bit for bit, also
+    # with a mass-flow inlet and a non-reflecting outlet; in FP64 that is the oracle's (the reference's) bits
+    say("all troubled", dims, sides, prec)
     nx, ny, nz = dims
     U = shock_state(nx, ny, nz, 9)
     extra = {}
     if sides == "inflow":
         extra = dict(INFLOW, **(dict(bc_zmin="periodic", bc_zmax="periodic") if nz > 1 else {}))
-    base = dict(base_keys(nx, ny, nz, reconstruction="weno5", flux_precision="fp32"), **extra)
-    lkeys = dict(learned_keys(nx, ny, nz), **extra)
+    base = dict(base_keys(nx, ny, nz, reconstruction="weno5", flux_precision=prec), **extra)
+    lkeys = dict(learned_keys(nx, ny, nz, prec), **extra)
     outs = []
     for keys, params in ((base, None), (lkeys, const_params(30.0)), (lkeys, None)):   # untrained: every face troubled
         with fvg.Solver(fvg.config_from_keys(keys)) as s:
@@ -145,6 +147,16 @@ def test_all_troubled_equals_fp32_weno(dims, sides):
     for R, S, Uo in outs[1:]:
         assert np.array_equal(R.view(np.uint64), outs[0][0].view(np.uint64)) and S == outs[0][1]
         assert np.array_equal(Uo.view(np.uint64), outs[0][2].view(np.uint64))
+    if prec == "fp64":
+        cfg = fvg.config_from_keys(base)
+        Ro, So = oracle.residual(cfg, U)
+        assert np.array_equal(outs[0][0].view(np.uint64), Ro.view(np.uint64)) and outs[0][1] == So
+        Uo, t = U.copy(), 0.0
+        for _ in range(3):
+            dt, _, _, ok = oracle.step(cfg, Uo, t, 100.0)
+            assert ok
+            t += dt
+        assert np.array_equal(outs[0][2].view(np.uint64), Uo.view(np.uint64))
 
 
 @pytest.mark.parametrize("dims", [(24, 20, 1), (16, 14, 12)])
@@ -174,13 +186,50 @@ def test_all_smooth_equals_oracle_upwind5(oracle, dims):
     assert abs(S - So) <= 1e-5 * So
 
 
-def test_audit_counts_and_fallback():
+@pytest.mark.parametrize("dims,sides", [((24, 20, 1), "periodic"), ((16, 14, 12), "periodic"), ((20, 18, 1), "inflow"),
+                                        ((14, 12, 10), "inflow")])
+def test_all_smooth_fp64_equals_oracle_upwind5_bitwise(oracle, dims, sides):
+    # FP64 fluxes, a network that calls every face smooth: residual, wave speed and three SSP-RK3 steps are the
+    # oracle's with the linear weights, bit for bit (the learned FP64 path is the reference's arithmetic everywhere but
+    # in the face values of the faces it calls smooth)
+    say("all smooth fp64", dims, sides)
+    nx, ny, nz = dims
+    extra = {}
+    if sides == "inflow":
+        extra = dict(INFLOW, **(dict(bc_zmin="periodic", bc_zmax="periodic") if nz > 1 else {}))
+    U = smooth_state(nx, ny, nz, 12, mach=0.3)
+    # no audit: on this noisy state the audit would find troubled faces the network calls smooth and switch it off
+    with fvg.Solver(fvg.config_from_keys(dict(learned_keys(nx, ny, nz, "fp64", indicator_audit="0"), **extra))) as s:
+        s.set_indicator(const_params(-30.0), 0.0)
+        s.upload(U)
+        R, S = s.residual()
+        s.advance(0.0, 100.0, max_steps=3)
+        Ud = s.download()
+    cfg = fvg.config_from_keys(dict(base_keys(nx, ny, nz, reconstruction="weno5"), **extra))
+    oracle.set_weno_linear(True)
+    try:
+        Ro, So = oracle.residual(cfg, U)
+        Uo, t = U.copy(), 0.0
+        for _ in range(3):
+            dt, _, _, ok = oracle.step(cfg, Uo, t, 100.0)
+            assert ok
+            t += dt
+    finally:
+        oracle.set_weno_linear(False)
+    Rw, _ = oracle.residual(cfg, U)
+    assert np.array_equal(R.view(np.uint64), Ro.view(np.uint64)) and S == So
+    assert np.array_equal(Ud.view(np.uint64), Uo.view(np.uint64))
+    assert not np.array_equal(Rw, Ro)   # WENO5-Z and upwind-5 differ on this state
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_audit_counts_and_fallback(prec):
     # on a shock state a network that calls every face smooth misses every troubled face: the audit of the first
-    # step switches the indicator off, and from then on the run is the FP32 WENO5-Z run
-    say("audit fallback")
+    # step switches the indicator off, and from then on the run is the WENO5-Z run of its flux precision
+    say("audit fallback", prec)
     nx, ny = 40, 32
     U = shock_state(nx, ny, 1, 11)
-    keys = learned_keys(nx, ny, 1, indicator_audit="1", indicator_fn_limit="0.01")
+    keys = learned_keys(nx, ny, 1, prec, indicator_audit="1", indicator_fn_limit="0.01")
     with fvg.Solver(fvg.config_from_keys(keys)) as s:
         s.set_indicator(const_params(-30.0), 0.0)
         s.upload(U)

@@ -244,8 +244,8 @@ def test_graph_replays_match_eager_steps(oracle, case):
 
 
 @pytest.mark.parametrize("case", ["weno5-visc-rk3", "muscl-rk2-walls", "fp32-weno5-rk3", "2d-weno5-visc-rk3",
-                                  "2d-muscl-rk2-walls", "fp32-learned-rk3", "2d-fp32-learned-rk3", "velin-outlet-last-axis",
-                                  "2d-inflow-nrbc-last-axis"])
+                                  "2d-muscl-rk2-walls", "fp32-learned-rk3", "2d-fp32-learned-rk3", "fp64-learned-rk3",
+                                  "2d-fp64-learned-rk3", "velin-outlet-last-axis", "2d-inflow-nrbc-last-axis"])
 def test_fused_stages_match_separate_passes(oracle, case):
     # the last axis' faces fused with the update (stages 2-3) give the bits of the separate face and update passes,
     # and (FP64) the oracle's
@@ -272,10 +272,10 @@ def test_fused_stages_match_separate_passes(oracle, case):
         keys = base_keys(*n, periodic=False, reconstruction="muscl", boundary_order="2", bc_xmin="periodic", bc_xmax="periodic",
                          bc_ymin="massflow_inlet", bc_ymax="nonreflecting", u_in="0.3", rho_in="0.9", p_out="1.1")
         U = smooth_state(*n, 46, mach=0.3)
-    elif case == "fp32-learned-rk3":
+    elif case in ("fp32-learned-rk3", "fp64-learned-rk3"):
         # the network's first-stage decisions are used by the fused last-axis faces as by the separate passes; the
         # trained network on a shock state gives both smooth and troubled faces
-        keys = base_keys(*n, reconstruction="weno5", flux_precision="fp32", indicator="learned", indicator_audit="0")
+        keys = base_keys(*n, reconstruction="weno5", flux_precision=case[:4], indicator="learned", indicator_audit="0")
         U = shock_state(*n, 44)
     else:
         keys = base_keys(*n, reconstruction="weno5", flux_precision="fp32", use_viscous="true", mu="0.01")
@@ -285,7 +285,7 @@ def test_fused_stages_match_separate_passes(oracle, case):
     for fused in (True, False):
         with fvg.Solver(cfg) as s:
             s.set_fused(fused)
-            if case == "fp32-learned-rk3":
+            if case.endswith("learned-rk3"):
                 from fenice_amd.train_indicator import DEFAULT_OUT, load
                 params, thr, _, _ = load(DEFAULT_OUT)
                 s.set_indicator(params, thr)
@@ -293,7 +293,7 @@ def test_fused_stages_match_separate_passes(oracle, case):
             t, k, _ = s.advance(0.0, 100.0, max_steps=4)
             out.append((s.download(), t))
     assert out[0][1] == out[1][1] and same_bits(out[0][0], out[1][0])
-    if case not in ("fp32-weno5-rk3", "fp32-learned-rk3"):
+    if case not in ("fp32-weno5-rk3", "fp32-learned-rk3", "fp64-learned-rk3"):
         Uo = U.copy()
         oracle.run(cfg, Uo, 100.0, max_steps=4)
         assert same_bits(out[0][0], Uo)
