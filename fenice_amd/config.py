@@ -18,6 +18,7 @@ TS_AUTO, TS_EULER, TS_SSPRK2, TS_SSPRK3, TS_HANCOCK = -1, 0, 1, 2, 3
 EOS_IDEAL, EOS_STIFFENED = 0, 1
 FLUX_FP64, FLUX_FP32 = 0, 1
 INDICATOR_NONE, INDICATOR_LEARNED = 0, 1
+LABEL_WENO5, LABEL_MUSCL, LABEL_BOTH = 0, 1, 2
 QUAD_MIDPOINT, QUAD_FOURTH = 0, 1
 
 # status codes
@@ -41,6 +42,7 @@ class Config(C.Structure):
         ("indicator_fn_limit", C.c_double), ("indicator_fn_floor", C.c_double),
         ("flux_quadrature", C.c_int),
         ("nrbc_sigma", C.c_double), ("nrbc_beta", C.c_double),
+        ("indicator_label_source", C.c_int),
     ]
 
     @staticmethod
@@ -64,6 +66,7 @@ class Config(C.Structure):
         c.indicator_fn_floor = 1e-4
         c.flux_quadrature = QUAD_MIDPOINT
         c.nrbc_sigma, c.nrbc_beta = 0.28, -1.0
+        c.indicator_label_source = LABEL_WENO5
         return c
 
     @staticmethod

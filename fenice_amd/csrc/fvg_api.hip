@@ -833,6 +833,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     p.ind = c->ind;
     p.ind_audit = cfg.indicator_audit;
     p.ind_label = (float)cfg.indicator_label;
+    p.ind_src = cfg.indicator_label_source;
     p.ind_fn_limit = cfg.indicator_fn_limit;
     p.ind_fn_floor = cfg.indicator_fn_floor;
     DevState s = fresh_state(c);

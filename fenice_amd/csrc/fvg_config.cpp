@@ -87,6 +87,7 @@ void fvg_config_defaults(fvg_config* c) {
     c->indicator_label = 0.05;
     c->indicator_fn_limit = 0.01;
     c->indicator_fn_floor = 1e-4;
+    c->indicator_label_source = FVG_LABEL_WENO5;
     c->flux_quadrature = FVG_QUAD_MIDPOINT;
     c->nrbc_sigma = 0.28;
     c->nrbc_beta = -1.0;
@@ -182,6 +183,12 @@ int fvg_config_set(fvg_config* c, const char* key_c, const char* value_c, char* 
     if (key == "indicator_label") return num(c->indicator_label);
     if (key == "indicator_fn_limit") return num(c->indicator_fn_limit);
     if (key == "indicator_fn_floor") return num(c->indicator_fn_floor);
+    if (key == "indicator_label_source") {
+        if (lv == "weno5" || lv == "weno") return (c->indicator_label_source = FVG_LABEL_WENO5), FVG_OK;
+        if (lv == "muscl") return (c->indicator_label_source = FVG_LABEL_MUSCL), FVG_OK;
+        if (lv == "both") return (c->indicator_label_source = FVG_LABEL_BOTH), FVG_OK;
+        return fail(err, n, "Unknown indicator_label_source: '" + v + "' (use weno5, muscl or both)");
+    }
     if (key == "nrbc_sigma") return num(c->nrbc_sigma);   // Params.cpp:274-275
     if (key == "nrbc_beta") return num(c->nrbc_beta);
     if (key == "flux_quadrature") {   // Params.cpp:348, 448-449

@@ -89,12 +89,13 @@ struct KP {
     // of the side's boundary cells, side s from element nrbc_off[s] (null without such a side)
     double* nrbc;
     long long nrbc_off[6];
-    // learned troubled-face indicator (FP32 fluxes, WENO5-Z)
+    // learned troubled-face indicator (WENO5-Z, either flux precision)
     unsigned char* ind_kept[3];   // per axis, one byte per face: the first stage's decision (1 troubled), kept for the step's later stages
     const MlpDev* mlp;
     IndState* ind;
     int ind_audit;            // steps between audits (0: none)
-    float ind_label;          // weight departure of a troubled face
+    float ind_label;          // label value above which a face is troubled
+    int ind_src;              // label source (enum fvg_label_source): WENO5-Z weight departure, MUSCL limiter clip, both
     double ind_fn_limit;      // missed fraction of troubled faces that switches the indicator off
     double ind_fn_floor;      // ... when the missed faces are also above this fraction of all audited faces
 };

@@ -1019,8 +1019,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     block_max_slot(p, local, SPD_CONV);
 }
 
-// Audit of the indicator on the stage input of an audit step: the network's decision against the label (WENO5-Z
-// weight departure above ind_label) on every face with a full stencil; counts TP, FP, FN, TN.
+// Audit of the indicator on the stage input of an audit step: the network's decision against the label (the label
+// value of the configured source above ind_label) on every face with a full stencil; counts TP, FP, FN, TN.
 template <int NV, int D, typename T>
 __global__ __launch_bounds__(256) void k_audit(KP p) {
     if (p.st->done || p.ind_audit <= 0 || (p.ind->steps % p.ind_audit) != 0) return;   // uniform
@@ -1039,7 +1039,7 @@ __global__ __launch_bounds__(256) void k_audit(KP p) {
 #pragma unroll
         for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
         face_features<NV, T>(c, float(p.weno_eps), x);
-        label = face_departure<NV, T>(c, T(p.weno_eps)) > T(p.ind_label);
+        label = face_label<NV, T>(c, T(p.weno_eps), p.ind_src, p.lim) > T(p.ind_label);
     }
     const float z = mlp_logit_wave(p.mlp, x);
     const bool pred = z > p.mlp->thr;
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_samples(KP p, float* __restrict__ X, fl
 #pragma unroll
         for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
         face_features<NV, T>(c, float(p.weno_eps), x);
-        d = float(face_departure<NV, T>(c, T(p.weno_eps)));
+        d = float(face_label<NV, T>(c, T(p.weno_eps), p.ind_src, p.lim));
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) X[f * 16 + q] = x[q];

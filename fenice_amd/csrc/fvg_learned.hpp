@@ -108,7 +108,7 @@ template <typename T> __device__ __forceinline__ T mean_square5(T c0, T c1, T c2
 // regularised by the face's scale mean(|v|^2 + p / rho) over its six cells rather than by its own mean square, so
 // that a component near zero (where rounding of the much larger pressure flux leaves noise of ~1e-7) does not count
 // as a discontinuity; density and pressure keep their own mean square, as WENO5-Z itself regularises them.
-template <int NV, typename T> __device__ __forceinline__ T face_departure(const T (*c)[NV], T eps_rel) {
+template <int NV, typename T> __device__ __forceinline__ T velocity_scale(const T (*c)[NV]) {
     T vs = T(0);
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
@@ -116,7 +116,10 @@ template <int NV, typename T> __device__ __forceinline__ T face_departure(const 
         if constexpr (NV == 5) v2 += c[s][4] * c[s][4];
         vs += v2 + dv(c[s][3], c[s][0]);
     }
-    vs = vs * T(1.0 / 6.0);
+    return vs * T(1.0 / 6.0);
+}
+
+template <int NV, typename T> __device__ __forceinline__ T face_departure(const T (*c)[NV], T eps_rel, T vs) {
     T d = T(0);
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -126,6 +129,49 @@ template <int NV, typename T> __device__ __forceinline__ T face_departure(const 
         d = smax(d, weno5z_departure(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], eps_rel, ml));
         d = smax(d, weno5z_departure(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], eps_rel, mr));
     }
+    return d;
+}
+
+// How far the MUSCL limiter moves the slope of one face value off the central slope: |phi(a, b) - (a + b)/2| over
+// sqrt((a^2 + b^2)/2 + eps_rel * m + 1e-30), with a the upwind and b the central difference as the reconstruction
+// takes them (apply_limiter, Schemes.hpp:110-117; wl = c1 + phi(c1 - c0, c2 - c1) / 2, Schemes.cpp:261). Zero on
+// linear data for every limiter (phi(a, a) = a).
+template <typename T> __device__ __forceinline__ T muscl_clip(int lim, T a, T b, T eps_rel, T m) {
+    const T phi = limiter(lim, a, b);
+    const T mid = T(0.5) * (a + b);
+    return dv(fabs(phi - mid), sqrt(T(0.5) * (a * a + b * b) + eps_rel * m + T(1e-30)));
+}
+
+template <typename T> __device__ __forceinline__ T mean_square4(T c0, T c1, T c2, T c3) {
+    return (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3) * T(0.25);
+}
+
+// clip label value of a face: the largest clip over the variables and both face values of the MUSCL stencil c1..c4
+// of the six-cell stencil (cells t-2 .. t+1 of the face between t-1 and t); the regularisation scale m as in
+// face_departure (a velocity component takes the face's kinetic-plus-acoustic scale, over the six cells)
+template <int NV, typename T> __device__ __forceinline__ T face_clip(const T (*c)[NV], T eps_rel, int lim, T vs) {
+    T d = T(0);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const bool vel = q == 1 || q == 2 || q == 4;
+        const T m = vel ? vs : mean_square4(c[1][q], c[2][q], c[3][q], c[4][q]);
+        const T b = c[3][q] - c[2][q];
+        d = smax(d, muscl_clip(lim, c[2][q] - c[1][q], b, eps_rel, m));   // left value: wl = c2 + phi(c2 - c1, c3 - c2) / 2
+        d = smax(d, muscl_clip(lim, c[4][q] - c[3][q], b, eps_rel, m));   // right value: wr = c3 - phi(c4 - c3, c3 - c2) / 2
+    }
+    return d;
+}
+
+// label sources (enum fvg_label_source of include/fvg.h)
+constexpr int LABEL_WENO5 = 0, LABEL_MUSCL = 1, LABEL_BOTH = 2;
+
+// label value of a face from its six-cell stencil: the WENO5-Z weight departure, the MUSCL limiter clip of the
+// configured limiter, or the larger of the two
+template <int NV, typename T> __device__ __forceinline__ T face_label(const T (*c)[NV], T eps_rel, int src, int lim) {
+    const T vs = velocity_scale<NV, T>(c);
+    T d = T(0);
+    if (src != LABEL_MUSCL) d = face_departure<NV, T>(c, eps_rel, vs);
+    if (src != LABEL_WENO5) d = smax(d, face_clip<NV, T>(c, eps_rel, lim, vs));
     return d;
 }
 

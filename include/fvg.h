@@ -59,14 +59,23 @@ enum fvg_eos { FVG_EOS_IDEAL = 0, FVG_EOS_STIFFENED = 1 };
  * conservation sums stay FP64. */
 enum fvg_flux_precision { FVG_FLUX_FP64 = 0, FVG_FLUX_FP32 = 1 };
 
-/* Troubled-face indicator of the WENO5-Z reconstruction (no reference key). LEARNED (with flux_precision = FP32
- * and reconstruction = weno5): a small network classifies every face from 15 features of its stencil; faces it
- * calls smooth take the linear upwind-5 value instead of WENO5-Z. An audit every indicator_audit steps compares its
- * decisions with the WENO5-Z weight departure (label: departure > indicator_label) and switches the indicator off
+/* Troubled-face indicator of the WENO5-Z reconstruction (no reference key). LEARNED (reconstruction = weno5, a
+ * Runge-Kutta scheme, the midpoint quadrature, either flux precision): a small network classifies every face from 15
+ * features of its stencil; faces it calls smooth take the linear upwind-5 value instead of WENO5-Z (in the flux
+ * precision: FP64 WENO5-Z is the reference's). An audit every indicator_audit steps compares its decisions with the
+ * label (the label value of indicator_label_source > indicator_label) and switches the indicator off
  * for the rest of the context's life when the missed troubled faces exceed both indicator_fn_limit of the troubled
  * faces and indicator_fn_floor of all the audited faces (the second keeps a handful of isolated label flips in a
  * smooth flow, e.g. where a velocity component crosses zero, from switching it off). */
 enum fvg_indicator { FVG_INDICATOR_NONE = 0, FVG_INDICATOR_LEARNED = 1 };
+/* Label sources of the indicator (key indicator_label_source): WENO5, the departure of the WENO5-Z weights from the
+ * linear weights (max over the variables and both face states of max_k |w_k - d_k|; weno5z, Schemes.cpp:338-353);
+ * MUSCL, how far the MUSCL limiter of the configured muscl_limiter (apply_limiter, Schemes.hpp:110-117, as the
+ * reconstruction applies it at Schemes.cpp:261-262) moves a face value's slope off the central slope,
+ * |phi(a, b) - (a + b) / 2| / sqrt((a^2 + b^2) / 2 + weno_epsilon * m + 1e-30) with a, b the one-sided differences
+ * and m the regularisation scale of the departure; BOTH, the larger of the two. A face is troubled when its label
+ * value exceeds indicator_label. */
+enum fvg_label_source { FVG_LABEL_WENO5 = 0, FVG_LABEL_MUSCL = 1, FVG_LABEL_BOTH = 2 };
 
 /* Flux quadrature over a face (flux_quadrature, Params.cpp:148, 448-455). FOURTH: WENO3-Z / WENO5-Z of the conserved
  * averages, point values and a fourth-order transverse correction of the face fluxes (Schemes.cpp:1954-2134); needs
@@ -102,6 +111,8 @@ typedef struct fvg_config {
     int flux_quadrature;         /* enum fvg_flux_quadrature (default MIDPOINT) */
     double nrbc_sigma;           /* non-reflecting outlet: relaxation of p towards p_out (default 0.28; 0: none) */
     double nrbc_beta;            /* non-reflecting outlet: transverse weight, <= 1 (default -1: the local Mach number) */
+    int indicator_label_source;  /* enum fvg_label_source: what makes a face troubled for the audit and the training
+                                    samples (no reference key; default WENO5) */
 } fvg_config;
 
 /* The reference's defaults (Params.cpp parameter_table): a 2-D 1x1 domain is not meaningful, so nx, ny,

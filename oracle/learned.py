@@ -10,6 +10,9 @@ The indicator has no reference counterpart beyond the functions its labels come 
   regularised as WENO5-Z regularises them (eps_rel times their mean square); velocity components by eps_rel times
   the face's mean |v|^2 + p / rho over its six cells, so that rounding noise in a component that is locally near
   zero does not count as a jump;
+- a second label source: where the MUSCL limiter clips, i.e. how far the limited slope phi(a, b) of a face value
+  (apply_limiter, Schemes.hpp:110-117) lies from the central slope (a + b) / 2, relative to the differences'
+  magnitude (clip); the label of a face may be the departure, the clip, or the larger of the two;
 - smooth faces take the linear upwind-5 value (2c0 - 13c1 + 47c2 + 27c3 - 3c4) / 60, which is weno5z with a = d
   (exact for quartic cell-average data);
 - features of a face (15): for density, pressure and the velocity component with the largest total variation over
@@ -102,6 +105,48 @@ def departure(st, eps_rel, nvel):
         d = np.maximum(np.abs(wl - D_LIN[:, None]).max(axis=0), np.abs(wr - D_LIN[:, None]).max(axis=0))
         out = np.maximum(out, d)
     return out
+
+
+def limiter(lim, a, b):
+    """The reference's slope limiters phi(a, b) (Schemes.hpp:82-117): 0 minmod, 1 van Leer, 2 MC, 3 Koren."""
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    same = a * b > 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if lim == 0:
+            return 0.5 * (np.sign(a) + np.sign(b)) * np.minimum(np.abs(a), np.abs(b))
+        if lim == 1:
+            return np.where(same, 2.0 * a * b / (a + b), 0.0)
+        if lim == 2:
+            return np.where(same, np.minimum(np.minimum(2.0 * np.abs(a), 0.5 * np.abs(a + b)), 2.0 * np.abs(b)) * np.sign(a), 0.0)
+        return np.where(same, np.minimum(np.minimum(2.0 * np.abs(a), np.abs(2.0 * a + b) / 3.0), 2.0 * np.abs(b)) * np.sign(a), 0.0)
+
+
+def clip(st, eps_rel, nvel, lim):
+    """MUSCL-limiter clip label value of each face: max over the variables and both face values of
+    |phi(a, b) - (a + b) / 2| / sqrt((a^2 + b^2) / 2 + eps_rel * m + 1e-30), on the four MUSCL cells c1..c4 of the
+    six-cell stencil; a = c2 - c1, b = c3 - c2 for the left value, a = c4 - c3, b = c3 - c2 for the right one (the
+    reconstruction's wl = c1 + phi(c1 - c0, c2 - c1) / 2, wr = c2 - phi(c3 - c2, c2 - c1) / 2, Schemes.cpp:261-262,
+    on cells t-2 .. t+1). m: the mean square of the four values (density, pressure) or the face's velocity scale."""
+    vs = velocity_scale(st, nvel)
+    vel = [1, 2] if nvel == 2 else [1, 2, 4]
+    out = np.zeros(st.shape[2])
+    for q in range(st.shape[1]):
+        c1, c2, c3, c4 = st[1, q], st[2, q], st[3, q], st[4, q]
+        m = vs if q in vel else (c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * 0.25
+        b = c3 - c2
+        for a in (c2 - c1, c4 - c3):
+            phi = limiter(lim, a, b)
+            out = np.maximum(out, np.abs(phi - 0.5 * (a + b)) / np.sqrt(0.5 * (a * a + b * b) + eps_rel * m + 1e-30))
+    return out
+
+
+def label_value(st, eps_rel, nvel, source="weno5", lim=1):
+    """The label value of each face: the WENO5-Z departure, the MUSCL clip, or the larger of the two ("both")."""
+    if source == "weno5":
+        return departure(st, eps_rel, nvel)
+    if source == "muscl":
+        return clip(st, eps_rel, nvel, lim)
+    return np.maximum(departure(st, eps_rel, nvel), clip(st, eps_rel, nvel, lim))
 
 
 def upwind5(c0, c1, c2, c3, c4):

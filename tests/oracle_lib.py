@@ -55,8 +55,14 @@ class Oracle:
         self.lib.fvo_face_flux_batch(C.byref(cfg), _dp(UL), _dp(UR), ax.ctypes.data_as(_I), n, _dp(F), _dp(S))
         return F, S
 
+    @staticmethod
+    def _check(cfg, U):
+        if U.ndim != 2 or U.shape[1] != 5 or U.shape[0] != cfg.nx * cfg.ny * cfg.nz:
+            raise ValueError(f"the oracle takes a ({cfg.nx * cfg.ny * cfg.nz}, 5) state; got {U.shape}")
+
     def residual(self, cfg, U, half_dt=0.0):
         U = np.ascontiguousarray(U, np.float64)
+        self._check(cfg, U)
         R = np.empty_like(U)
         S = C.c_double()
         rev = C.c_longlong()
@@ -77,6 +83,9 @@ class Oracle:
 
     def step(self, cfg, U, t, t_max, dt_max=0.0, lagged=0.0):
         """One step in place on U; returns (dt, s_max, lagged_speed, ok)."""
+        self._check(cfg, U)
+        if not (U.flags.c_contiguous and U.dtype == np.float64):
+            raise ValueError("the oracle steps a C-contiguous float64 state in place")
         lag = C.c_double(lagged)
         dt = C.c_double()
         S = C.c_double()

@@ -132,6 +132,15 @@ def test_reference_case_keys_outside_the_path():
         fvg.config_from_keys(dict(nx=8, ny=8, use_ibm="maybe"))
 
 
+def test_label_sources():
+    from fenice_amd.config import LABEL_BOTH, LABEL_MUSCL, LABEL_WENO5
+    for v, want in (("weno5", LABEL_WENO5), ("MUSCL", LABEL_MUSCL), ("both", LABEL_BOTH)):
+        assert fvg.config_from_keys(dict(nx=4, ny=4, indicator_label_source=v)).indicator_label_source == want
+    assert fvg.config_from_keys(dict(nx=4, ny=4)).indicator_label_source == LABEL_WENO5
+    with pytest.raises(ValueError, match="Unknown indicator_label_source"):
+        fvg.config_from_keys(dict(nx=4, ny=4, indicator_label_source="tvd"))
+
+
 def test_learned_indicator_in_both_flux_precisions():
     for prec in ("fp64", "fp32"):
         c = fvg.config_from_keys(dict(nx=4, ny=4, indicator="learned", reconstruction="weno5", flux_precision=prec))

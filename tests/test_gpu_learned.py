@@ -85,12 +85,16 @@ def prim_planes(U, nx, ny, nz, gamma=1.4, prec="fp32"):
     return W.astype(np.float32).astype(np.float64) if prec == "fp32" else W   # the path's primitives
 
 
+@pytest.mark.parametrize("source,lim", [("weno5", 1), ("muscl", 2), ("both", 1), ("both", 3)])
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("dims", [(24, 20, 1), (16, 14, 12)])
-def test_samples_match_cpu(dims, prec):
-    say("samples", dims, prec)
+def test_samples_match_cpu(dims, prec, source, lim):
+    # features and label values (WENO5-Z departure, MUSCL clip of the configured limiter, or both) of every face
+    say("samples", dims, prec, source, lim)
     nx, ny, nz = dims
-    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_precision=prec)
+    lname = {0: "Minmod", 1: "Van Leer", 2: "MC", 3: "Koren"}[lim]
+    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_precision=prec, indicator_label_source=source,
+                     muscl_limiter=lname)
     U = shock_state(nx, ny, nz, 8)
     W = prim_planes(U, nx, ny, nz, prec=prec)
     nv = 5 if nz > 1 else 4
@@ -106,7 +110,7 @@ def test_samples_match_cpu(dims, prec):
             Xd = Xd[:nz, :ny, :nx].reshape(-1, 16)
             dd = dd[:nz, :ny, :nx].reshape(-1)
             f = L.features(st, 1e-6, nv - 2)
-            dep = L.departure(st, 1e-6, nv - 2)
+            dep = L.label_value(st, 1e-6, nv - 2, source, lim)
             assert np.abs(Xd[:, :15] - f).max() < 2e-4
             assert np.all(Xd[:, 15] == 1.0)
             big = dep > 1e-3
@@ -333,3 +337,55 @@ def test_audit_switches_the_network_off_when_it_misses():
     tp, fp, fn, tn = total
     say(f"four quadrants, audit every step: audits {audits}, switched off {off}, tp/fp/fn/tn {total}")
     assert off and audits >= 1 and fn > 0 and tp + fn > 0
+
+
+def test_kh_learned_over_the_rollup(oracle):
+    # Kelvin-Helmholtz 256^2 (the layer of the 2048^2 workload), 800 steps at a fixed dt (t = 0.44: the sin(4 pi x)
+    # mode grows about 15-fold and the layer rolls up; its density jumps are troubled faces from the start). The
+    # oracle's FP64 WENO5-Z run is the reference; the plain FP64 device run is its bits; the learned indicator (trained
+    # network, audit every 50 steps) with FP64 and FP32 fluxes, and the plain FP32 path, stay within 1 % relative L1
+    # density error of it. The L1 / Linf errors and the audits' counts are printed.
+    from fenice_amd.train_indicator import load
+    from fenice_amd.workloads import kelvin_helmholtz, kh_keys
+    n, steps, dt = 256, 800, 5.5e-4   # the CFL step of this run stays above 7e-4
+    params, thr, _, _ = load()
+    U = kelvin_helmholtz(n)
+    say("kh 256^2 roll-up: oracle FP64 WENO5-Z")
+    cfg = fvg.config_from_keys(kh_keys(n))
+    Uo = np.zeros((n * n, 5))
+    Uo[:, :4] = U
+    t = 0.0
+    for _ in range(steps):
+        d, _, _, ok = oracle.step(cfg, Uo, t, 100.0, dt)
+        assert ok and d == dt
+        t += d
+    rho_o = Uo[:, 0]
+    out = {}
+    for flux, learned in (("fp64", False), ("fp64", True), ("fp32", False), ("fp32", True)):
+        name = flux + ("+learned" if learned else "")
+        keys = dict(kh_keys(n), flux_precision=flux)
+        if learned:
+            keys.update(indicator="learned", indicator_audit="50")
+        with fvg.Solver(fvg.config_from_keys(keys)) as s:
+            if learned:
+                s.set_indicator(params, thr)
+            s.upload(U)
+            td, k, _ = s.advance(0.0, 100.0, max_steps=steps, dt_max=dt)
+            Ud = s.download()
+            st = s.indicator_stats() if learned else None
+        assert k == steps and td == t
+        rho = Ud[:, 0]
+        l1 = float(np.abs(rho - rho_o).sum() / np.abs(rho_o).sum())
+        linf = float(np.abs(rho - rho_o).max() / np.abs(rho_o).max())
+        out[name] = (l1, linf)
+        msg = f"{name}: density L1 {l1:.3e}, Linf {linf:.3e} against the oracle at t = {t:.4f}"
+        if st is not None:
+            _, total, off, audits = st
+            tp, fp, fn, tn = total
+            msg += (f"; {audits} audits, tp/fp/fn/tn {total}, troubled {(tp + fn) / sum(total):.4f} of the audited faces, "
+                    f"missed {fn / max(1, tp + fn):.4f} of them, switched off: {off}")
+        say(msg)
+        if name == "fp64":
+            assert np.array_equal(np.ascontiguousarray(Ud[:, :4]).view(np.uint64), np.ascontiguousarray(Uo[:, :4]).view(np.uint64))
+        else:
+            assert l1 < 1e-2, (name, l1)

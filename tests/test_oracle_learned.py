@@ -1,5 +1,6 @@
 """Pins of the learned indicator's CPU restatement (oracle/learned.py): gradient against central differences,
-upwind-5 exactness, the WENO5-Z departure at its closed-form cases, feature invariances, bfloat16 rounding."""
+upwind-5 exactness, the WENO5-Z departure and the MUSCL clip at their closed-form cases, the limiters' values,
+feature invariances, bfloat16 rounding."""
 import os
 import sys
 
@@ -40,6 +41,39 @@ def test_departure_closed_forms():
     assert L.departure(step, 1e-6, 3).max() > 0.25              # a jump concentrates the weights
     flat = np.full((6, 5, 1), 2.0)
     assert L.departure(flat, 1e-6, 3).max() == 0.0
+
+
+def test_limiters_closed_forms():
+    # phi(1, 3) of the four limiters (Schemes.hpp:82-110): minmod 1, van Leer 2*3/4, MC min(2, 2, 6), Koren min(2, 5/3, 6);
+    # opposite signs give 0, phi(a, a) = a, and phi is odd
+    want = {0: 1.0, 1: 1.5, 2: 2.0, 3: 5.0 / 3.0}
+    for lim, w in want.items():
+        assert abs(float(L.limiter(lim, 1.0, 3.0)) - w) < 1e-15, lim
+        assert float(L.limiter(lim, 1.0, -3.0)) == 0.0 and float(L.limiter(lim, 0.0, 2.0)) == 0.0
+        assert float(L.limiter(lim, 0.5, 0.5)) == 0.5
+        assert float(L.limiter(lim, -1.0, -3.0)) == -w
+
+
+def test_clip_closed_forms():
+    # the MUSCL clip label: zero on linear data for every limiter; a jump: the limited slope of the face value next to
+    # it is far from the central one; a symmetric extremum (a = -b): phi = 0 = (a + b)/2, no clip
+    j = np.arange(6, dtype=float)
+    lin = (1.3 + 0.07 * j)[:, None, None] * np.ones((6, 5, 1))
+    step = np.where(j < 3, 1.0, 0.125)[:, None, None] * np.ones((6, 5, 1))
+    for lim in range(4):
+        assert L.clip(lin, 1e-6, 3, lim).max() < 1e-12, lim
+        assert L.clip(step, 1e-6, 3, lim).max() > 0.25, lim
+    peak = np.array([1.0, 1.0, 2.0, 1.0, 1.0, 1.0])[:, None, None] * np.ones((6, 5, 1))   # c2 - c1 = -(c3 - c2)
+    c = peak[:, 0, 0]
+    a, b = c[2] - c[1], c[3] - c[2]
+    assert a == -b and L.clip(peak[:, :, :], 1e-6, 3, 2)[0] > 0.0   # the right value's pair (c4 - c3, b) does clip
+    only_left = L.limiter(2, a, b) - 0.5 * (a + b)
+    assert only_left == 0.0
+    # "both" is the larger of the two sources
+    rng = np.random.default_rng(5)
+    st = 1.0 + 0.3 * rng.random((6, 5, 50))
+    both = L.label_value(st, 1e-6, 3, "both", 2)
+    assert np.array_equal(both, np.maximum(L.departure(st, 1e-6, 3), L.clip(st, 1e-6, 3, 2)))
 
 
 def test_departure_ignores_rounding_noise_in_a_near_zero_velocity():
