@@ -43,6 +43,9 @@ from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
 from fenice_amd.workloads import taylor_green_keys as run_keys  # noqa: E402
 
 
+PASSES = "default"   # kernel sequence of the residual: the context's default, "fused" or "separate" (--passes)
+
+
 def make_solver(n: int, flux: str, indicator: str = "none", weights: str = ""):
     """The solver of the workload; with indicator = learned, the trained network's weights are loaded."""
     from fenice_amd import fvg
@@ -50,6 +53,8 @@ def make_solver(n: int, flux: str, indicator: str = "none", weights: str = ""):
     if indicator == "learned":
         keys.update(indicator="learned", indicator_audit="50")
     s = fvg.Solver(fvg.config_from_keys(keys))
+    if PASSES != "default":
+        s.set_fused(PASSES == "fused")
     if indicator == "learned":
         from fenice_amd.train_indicator import DEFAULT_OUT, load
         params, thr, _, _ = load(weights or DEFAULT_OUT)
@@ -468,6 +473,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="end with a 1 GiB device copy (counter calibration)")
     ap.add_argument("--flux", default="fp64", choices=["fp64", "fp32"], help="arithmetic of reconstruction and fluxes")
     ap.add_argument("--indicator", default="none", choices=["none", "learned"], help="troubled-face indicator (WENO5-Z, either flux precision)")
+    ap.add_argument("--passes", default="default", choices=["default", "fused", "separate"],
+                    help="kernel sequence of the residual: fused face passes (no face fluxes in HBM) or separate face and update passes")
     ap.add_argument("--weights", default="", help="trained indicator weights (default fenice_amd/indicator_weights.npz)")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32 / learned-indicator lines of the FP64 run")
     ap.add_argument("--quadrature", default="midpoint", choices=["midpoint", "fourth"], help="flux quadrature of the CPU worker")
@@ -477,6 +484,8 @@ def main():
     ap.add_argument("--side", default="", choices=["", "ensemble", "vortex", "kh", "tgv512", "fourth"],
                     help="measure one of the other configurations instead (prints its own JSON line)")
     a = ap.parse_args()
+    global PASSES
+    PASSES = a.passes
     if a.cpu_worker:
         if a.workload == "ensemble":   # --n: processes, --cpu-budget: steps per instance
             print(json.dumps(cpu_ensemble_worker(a.n, int(a.cpu_budget))))

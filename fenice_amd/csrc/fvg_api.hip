@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -24,7 +25,9 @@ template <int NV, int D, int MODE, typename T> __global__ void k_faces(KP);
 template <int NV, int UPD, typename T> __global__ void k_update(KP, const double*, const double*, double*);
 template <int NV, int WHAT> __global__ void k_totals_partial(const double*, long long, double*);
 template <int NV, int D, int USE, typename T> __global__ void k_faces_learned(KP);
-template <int NV, int MODE, int UPD, typename T> __global__ void k_zupdate(KP, const double*, const double*, double*);
+template <int NV, int MODE, int USE, typename T> __global__ void k_xacc(KP);
+template <int NV, int D, int MODE, int USE, int UPD, typename T> __global__ void k_lacc(KP, const double*, const double*, double*);
+template <int NV> __global__ void k_axpy(KP, const double*, const double*, double*);
 template <int NV, int D, typename T> __global__ void k_audit(KP);
 template <int NV, int D, typename T> __global__ void k_samples(KP, float*, float*);
 template <int NV, int D, int R> __global__ void k4_states(KP);
@@ -45,12 +48,11 @@ namespace {
 constexpr int kBlock = 256;
 static_assert(kBlock >= kRowBlock, "block sizes of the kernels' launch bounds");
 constexpr int kGraphSteps = 4;   // steps per captured graph of fvg_advance
-constexpr int kZTileI = 64, kZTileK = 16;   // k_zupdate tile (faces): 64 along x, 16 along z
 thread_local std::string g_create_error;
 
-enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_ZUPDATE, FAM_AUDIT,
+enum Family { FAM_CELLS, FAM_HANCOCK, FAM_FACES_X, FAM_FACES_Y, FAM_FACES_Z, FAM_UPDATE, FAM_CONTROL, FAM_AXPY, FAM_AUDIT,
               FAM_Q4_STATES, FAM_Q4_POINTS, FAM_Q4_FLUXES, FAM_N };
-const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_zupdate",
+const char* kFamilyNames[FAM_N] = {"k_cells", "k_hancock", "k_faces_x", "k_faces_y", "k_faces_z", "k_update", "k_control", "k_axpy",
                                    "k_audit", "k4_states", "k4_points", "k4_fluxes"};
 
 struct Err {
@@ -91,7 +93,9 @@ struct fvg_ctx {
     MlpDev* mlp = nullptr;
     IndState* ind = nullptr;
     bool graphs = true;                // fvg_advance replays a captured graph of steps
-    bool fused = false;                // Runge-Kutta stages 2-3: last-axis faces and update in one pass (k_zupdate)
+    bool fused = false;                // fused residual passes (k_xacc, k_lacc; no face fluxes in HBM, opt-in), or
+                                       // separate face and update passes (k_faces, k_update; the default)
+    double* A = nullptr;               // KP::A
     hipGraphExec_t graph = nullptr;    // kGraphSteps steps of the scheme
     bool quad4 = false;                // flux_quadrature = fourth
     bool massflow = false;             // a mass-flow inlet side: k_massflow runs before every k_cells
@@ -341,65 +345,112 @@ int enqueue_update(fvg_ctx* c, int upd, const double* U0, const double* Ut, doub
     return c->NV == 5 ? enqueue_update_nv<5, double>(c, upd, U0, Ut, out) : enqueue_update_nv<4, double>(c, upd, U0, Ut, out);
 }
 
-// A Runge-Kutta stage after the first (dt known): cells, the faces of every axis but the last, then the last axis's
-// faces (z in 3-D, y in 2-D) fused with the update.
+// ---- fused residual passes (KP::A) ----
+// the (mode, use, upd) combinations the device units instantiate: MUSCL-Hancock and the learned network's first
+// stage only store R (UPD_RESIDUAL; the update is k_axpy), the kept decisions of the learned indicator only serve
+// the stages whose dt is known; upd < 0 is the y pass of a 3-D mesh
+template <int MODE, int USE, int UPD> constexpr bool fused_combo() {
+    if (MODE == MODE_HANCOCK) return USE == 0 && UPD <= UPD_RESIDUAL;
+    if (MODE == MODE_LEARNED) return USE == LF_STORE ? UPD <= UPD_RESIDUAL : UPD != UPD_RESIDUAL;
+    return USE == 0;
+}
+
+unsigned xacc_grid(const fvg_ctx* c) {   // 4 wavefronts per block, 63 cells per wavefront
+    const long long waves = (c->nfa[0] - 1 + 62) / 63;
+    return (unsigned)((waves + 3) / 4);
+}
+unsigned lacc_grid(const fvg_ctx* c, int d) {   // 4 wavefronts per block on 64 x positions of one line segment
+    return (unsigned)((long long)c->base.nlchunk * c->base.nseg[d] * (d == 1 ? c->base.nz : c->base.ny));
+}
+
+template <int NV, int D, int MODE, int USE, int UPD, typename T>
+void launch_lacc(fvg_ctx* c, const KP& p, const double* U0, const double* Ut, double* out) {
+    if constexpr (fused_combo<MODE, USE, UPD>())
+        hipLaunchKernelGGL((k_lacc<NV, D, MODE, USE, UPD, T>), dim3(lacc_grid(c, D)), dim3(kBlock), 0, c->stream, p, U0, Ut, out);
+}
+
+// x faces into the accumulators, y faces added (3-D), then the last axis finishes R: stored into out (UPD_RESIDUAL)
+// or combined into the stage output (dt known)
+template <int NV, int MODE, int USE, typename T>
+int enqueue_fused_faces(fvg_ctx* c, const KP& p, int upd, const double* U0, const double* Ut, double* out) {
+    constexpr int DL = NV == 5 ? 2 : 1;
+    if (!((upd == UPD_RESIDUAL && fused_combo<MODE, USE, UPD_RESIDUAL>()) || (upd != UPD_RESIDUAL && fused_combo<MODE, USE, UPD_RK3_3>())))
+        return set_err(c, FVG_ELOGIC, "no fused pass for this mode and stage");
+    int rc;
+    if ((rc = launch(c, FAM_FACES_X, [&] {
+             hipLaunchKernelGGL((k_xacc<NV, MODE, USE, T>), dim3(xacc_grid(c)), dim3(kBlock), 0, c->stream, p);
+         })))
+        return rc;
+    if constexpr (NV == 5)
+        if ((rc = launch(c, FAM_FACES_Y, [&] { launch_lacc<NV, 1, MODE, USE, -1, T>(c, p, nullptr, nullptr, nullptr); }))) return rc;
+    return launch(c, DL == 2 ? FAM_FACES_Z : FAM_FACES_Y, [&] {
+        switch (upd) {
+        case UPD_RESIDUAL: launch_lacc<NV, DL, MODE, USE, UPD_RESIDUAL, T>(c, p, U0, Ut, out); break;
+        case UPD_RK2_2: launch_lacc<NV, DL, MODE, USE, UPD_RK2_2, T>(c, p, U0, Ut, out); break;
+        case UPD_RK3_2: launch_lacc<NV, DL, MODE, USE, UPD_RK3_2, T>(c, p, U0, Ut, out); break;
+        default: launch_lacc<NV, DL, MODE, USE, UPD_RK3_3, T>(c, p, U0, Ut, out); break;
+        }
+    });
+}
+
+// One fused residual of Uin: cells (and the MUSCL-Hancock predictor, the indicator's audit), then the face passes.
+// reuse: the learned indicator's faces take the decisions of the step's first stage.
 template <int NV, typename T>
-int enqueue_fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
+int enqueue_fused_nv(fvg_ctx* c, const double* Uin, int mode, bool audit, bool reuse, int upd, const double* U0, const double* Ut,
+                     double* out) {
     const KP p = make_kp(c, Uin);
     int rc;
     if ((rc = enqueue_massflow(c, p))) return rc;
     if ((rc = launch(c, FAM_CELLS, [&] { hipLaunchKernelGGL((k_cells<NV, T>), dim3(cells_grid(c)), dim3(cells_block(c)), 0, c->stream, p); })))
         return rc;
-    for (int d = 0; d < (NV == 5 ? 2 : 1); ++d) {
-        rc = launch(c, FAM_FACES_X + d, [&] {
-            const dim3 g(grid_for(c->nfa[d])), b(kBlock);
-            if (mode == MODE_LEARNED) {   // stages 2-3: the first stage's decisions
-                if (d == 0) hipLaunchKernelGGL((k_faces_learned<NV, 0, LF_KEPT, T>), g, b, 0, c->stream, p);
-                else hipLaunchKernelGGL((k_faces_learned<NV, 1, LF_KEPT, T>), g, b, 0, c->stream, p);
-                return;
-            }
-            if (mode == MODE_PLAIN) {
-                if (d == 0) hipLaunchKernelGGL((k_faces<NV, 0, MODE_PLAIN, T>), g, b, 0, c->stream, p);
-                else hipLaunchKernelGGL((k_faces<NV, 1, MODE_PLAIN, T>), g, b, 0, c->stream, p);
-            } else {
-                if (d == 0) hipLaunchKernelGGL((k_faces<NV, 0, MODE_FIRST_ORDER, T>), g, b, 0, c->stream, p);
-                else hipLaunchKernelGGL((k_faces<NV, 1, MODE_FIRST_ORDER, T>), g, b, 0, c->stream, p);
-            }
-        });
-        if (rc) return rc;
-    }
-    const long long nit = (c->cfg.nx + kZTileI - 1) / kZTileI;
-    const long long nlast = NV == 5 ? c->cfg.nz : c->cfg.ny;
-    const long long blocks = nit * (NV == 5 ? c->cfg.ny : 1) * ((nlast + kZTileK - 2) / (kZTileK - 1));
-    const dim3 g((unsigned)blocks), b(kZTileI * kZTileK);
-    return launch(c, FAM_ZUPDATE, [&] {
-#define ZU(M, U) hipLaunchKernelGGL((k_zupdate<NV, M, U, T>), g, b, 0, c->stream, p, U0, Ut, out)
-#define ZU_MODE(M)                           \
-    switch (upd) {                           \
-    case UPD_RK2_2: ZU(M, UPD_RK2_2); break; \
-    case UPD_RK3_2: ZU(M, UPD_RK3_2); break; \
-    default: ZU(M, UPD_RK3_3); break;        \
-    }
-        if (mode == MODE_LEARNED) {
-            ZU_MODE(MODE_LEARNED)
-            return;
+    if constexpr (std::is_same<T, double>::value) {
+        if (mode == MODE_HANCOCK) {
+            if ((rc = launch(c, FAM_HANCOCK, [&] { hipLaunchKernelGGL(k_hancock<NV>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p); })))
+                return rc;
+            return enqueue_fused_faces<NV, MODE_HANCOCK, 0, T>(c, p, upd, U0, Ut, out);
         }
-        if (mode == MODE_PLAIN) {
-            ZU_MODE(MODE_PLAIN)
-        } else {
-            ZU_MODE(MODE_FIRST_ORDER)
+    }
+    if (mode == MODE_LEARNED) {
+        if (audit) {
+#define AUDIT(D) launch(c, FAM_AUDIT, [&] { hipLaunchKernelGGL((k_audit<NV, D, T>), dim3(grid_for(c->nfa[D])), dim3(kBlock), 0, c->stream, p); })
+            if ((rc = AUDIT(0)) || (rc = AUDIT(1))) return rc;
+            if constexpr (NV == 5)
+                if ((rc = AUDIT(2))) return rc;
+#undef AUDIT
         }
-#undef ZU_MODE
-#undef ZU
+        return reuse ? enqueue_fused_faces<NV, MODE_LEARNED, LF_KEPT, T>(c, p, upd, U0, Ut, out)
+                     : enqueue_fused_faces<NV, MODE_LEARNED, LF_STORE, T>(c, p, upd, U0, Ut, out);
+    }
+    if (mode == MODE_PLAIN) return enqueue_fused_faces<NV, MODE_PLAIN, 0, T>(c, p, upd, U0, Ut, out);
+    if (mode == MODE_FIRST_ORDER) return enqueue_fused_faces<NV, MODE_FIRST_ORDER, 0, T>(c, p, upd, U0, Ut, out);
+    return set_err(c, FVG_ELOGIC, "no fused pass for this mode");
+}
+
+int enqueue_fused(fvg_ctx* c, const double* Uin, int mode, bool audit, bool reuse, int upd, const double* U0, const double* Ut,
+                  double* out) {
+    if (c->fp32)
+        return c->NV == 5 ? enqueue_fused_nv<5, float>(c, Uin, mode, audit, reuse, upd, U0, Ut, out)
+                          : enqueue_fused_nv<4, float>(c, Uin, mode, audit, reuse, upd, U0, Ut, out);
+    return c->NV == 5 ? enqueue_fused_nv<5, double>(c, Uin, mode, audit, reuse, upd, U0, Ut, out)
+                      : enqueue_fused_nv<4, double>(c, Uin, mode, audit, reuse, upd, U0, Ut, out);
+}
+
+int enqueue_axpy(fvg_ctx* c, const double* U0, const double* R, double* out) {
+    const KP p = c->base;
+    return launch(c, FAM_AXPY, [&] {
+        if (c->NV == 5) hipLaunchKernelGGL((k_axpy<5>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p, U0, R, out);
+        else hipLaunchKernelGGL((k_axpy<4>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, p, U0, R, out);
     });
 }
 
-int fused_stage(fvg_ctx* c, const double* Uin, int mode, int upd, const double* U0, const double* Ut, double* out) {
-    if (c->NV == 5)
-        return c->fp32 ? enqueue_fused_stage<5, float>(c, Uin, mode, upd, U0, Ut, out)
-                       : enqueue_fused_stage<5, double>(c, Uin, mode, upd, U0, Ut, out);
-    return c->fp32 ? enqueue_fused_stage<4, float>(c, Uin, mode, upd, U0, Ut, out)
-                   : enqueue_fused_stage<4, double>(c, Uin, mode, upd, U0, Ut, out);
+// whether the residuals of this context take the fused passes (the fourth-order quadrature keeps its own passes)
+bool fused_path(const fvg_ctx* c) { return c->fused && !c->quad4; }
+
+// The residual of Uin stored into R (Rb for fvg_residual): fused passes, or separate face and update passes
+int residual_into(fvg_ctx* c, const double* Uin, int mode, bool audit, double* R) {
+    if (fused_path(c)) return enqueue_fused(c, Uin, mode, audit, false, UPD_RESIDUAL, nullptr, nullptr, R);
+    int rc = enqueue_residual(c, Uin, mode, audit);
+    return rc ? rc : enqueue_update(c, UPD_RESIDUAL, nullptr, nullptr, R);
 }
 
 int plain_mode(const fvg_ctx* c) {
@@ -415,19 +466,24 @@ enum { LINK_AFTER = 1, LINK_BEFORE = 2 };
 int enqueue_step(fvg_ctx* c, int link = 0) {
     int rc;
     const int ts = c->cfg.time_scheme;
+    const bool fused = fused_path(c);
     if (ts == FVG_TS_HANCOCK) {
         const int mode = c->cfg.reconstruction == FVG_RECON_NONE ? MODE_FIRST_ORDER : MODE_HANCOCK;
         if (!c->lagged_positive) {   // no wave speed yet: one plain residual gives it (TimeScheme.cpp:298-302)
             link = 0;
             if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
-            if ((rc = enqueue_residual(c, c->U, plain_mode(c)))) return rc;
+            if ((rc = fused ? residual_into(c, c->U, plain_mode(c), false, c->Rb) : enqueue_residual(c, c->U, plain_mode(c)))) return rc;
             if ((rc = enqueue_control(c, CTL_SEED, ts))) return rc;
             if ((rc = enqueue_control(c, CTL_HANCOCK_DT, ts))) return rc;
         } else if ((rc = enqueue_control(c, (link & LINK_AFTER) ? CTL_END_BEGIN_HANCOCK : CTL_BEGIN_HANCOCK, ts))) {
             return rc;
         }
-        if ((rc = enqueue_residual(c, c->U, mode))) return rc;
-        if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
+        if (fused) {   // the residual into Rb, then U += dt Rb (the faces read U, so the update waits for them)
+            if ((rc = residual_into(c, c->U, mode, false, c->Rb)) || (rc = enqueue_axpy(c, c->U, c->Rb, c->U))) return rc;
+        } else {
+            if ((rc = enqueue_residual(c, c->U, mode))) return rc;
+            if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->U))) return rc;
+        }
         if (!(link & LINK_BEFORE) && (rc = enqueue_control(c, CTL_END_HANCOCK, ts))) return rc;
         c->lagged_positive = true;
         return FVG_OK;
@@ -435,6 +491,24 @@ int enqueue_step(fvg_ctx* c, int link = 0) {
     if ((rc = enqueue_control(c, CTL_BEGIN_STEP, ts))) return rc;
     const int mode = plain_mode(c);
     if (c->learned && (rc = enqueue_indicator(c, CTL_AUDIT_BEGIN))) return rc;
+    if (fused) {
+        // stage 1: R into Rb, dt from its wave speed, then the first update; stages 2-3 finish their combination in
+        // the last face pass (the SSP-RK3 stage-2 output goes to Rb, which stage 1's R no longer needs)
+        if ((rc = residual_into(c, c->U, mode, c->learned, c->Rb))) return rc;
+        if ((rc = enqueue_control(c, CTL_STAGE1_DT, ts))) return rc;
+        if ((rc = enqueue_axpy(c, c->U, c->Rb, ts == FVG_TS_EULER ? c->U : c->Ut))) return rc;
+        if (ts == FVG_TS_SSPRK2) {
+            if ((rc = enqueue_fused(c, c->Ut, mode, false, c->learned, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
+        } else if (ts == FVG_TS_SSPRK3) {
+            if ((rc = enqueue_fused(c, c->Ut, mode, false, c->learned, UPD_RK3_2, c->U, c->Ut, c->Rb))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
+            if ((rc = enqueue_fused(c, c->Rb, mode, false, c->learned, UPD_RK3_3, c->U, c->Rb, c->U))) return rc;
+            if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
+        }
+        if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
+        return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
+    }
     if ((rc = enqueue_residual(c, c->U, mode, c->learned))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE1_DT, ts))) return rc;
     if (ts == FVG_TS_EULER) {
@@ -443,19 +517,6 @@ int enqueue_step(fvg_ctx* c, int link = 0) {
         return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
     }
     if ((rc = enqueue_update(c, UPD_AXPY, c->U, nullptr, c->Ut))) return rc;
-    if (c->fused && mode != MODE_FOURTH) {   // stages 2-3: the last axis' faces fused with the update; the SSP-RK3 stage-2 output goes to Rb
-        if (ts == FVG_TS_SSPRK2) {
-            if ((rc = fused_stage(c, c->Ut, mode, UPD_RK2_2, c->U, c->Ut, c->U))) return rc;
-            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
-        } else {
-            if ((rc = fused_stage(c, c->Ut, mode, UPD_RK3_2, c->U, c->Ut, c->Rb))) return rc;
-            if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
-            if ((rc = fused_stage(c, c->Rb, mode, UPD_RK3_3, c->U, c->Rb, c->U))) return rc;
-            if ((rc = enqueue_control(c, CTL_STAGE_S, 3))) return rc;
-        }
-        if ((rc = enqueue_control(c, CTL_END_RK, ts))) return rc;
-        return c->learned ? enqueue_indicator(c, CTL_AUDIT_END) : FVG_OK;
-    }
     if ((rc = enqueue_residual(c, c->Ut, mode, false, c->learned))) return rc;
     if ((rc = enqueue_control(c, CTL_STAGE_S, 2))) return rc;
     if (ts == FVG_TS_SSPRK2) {
@@ -527,7 +588,7 @@ void free_ctx(fvg_ctx* c) {
     if (c->graph) hipGraphExecDestroy(c->graph);
     hipFree(c->mlp);
     hipFree(c->ind);
-    void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,         c->HS,  c->Fc[0], c->Fc[1], c->Fc[2],
+    void* bufs[] = {c->U,     c->Ut,    c->Rb,    c->W,        c->G,         c->HS,  c->Fc[0], c->Fc[1], c->Fc[2], c->A,
                     c->Fv[0], c->Fv[1], c->Fv[2], c->tot_part, c->tot_out, c->spd, c->Wc,    c->q4_UL, c->q4_UR,
                     c->q4_Fp, c->q4_good, c->q4_pt, c->nrbc, c->ind_kept[0], c->ind_kept[1], c->ind_kept[2]};
     for (void* b : bufs)
@@ -552,6 +613,18 @@ int checked_alloc(fvg_ctx* c, P** p, size_t n, size_t elem = sizeof(double)) {
         return set_err(c, FVG_ENOMEM, "hipMalloc of " + std::to_string(n * elem) + " bytes: " + hipGetErrorString(e));
     }
     return FVG_OK;
+}
+
+// the face-flux planes of the separate face and update passes: per axis NV convective planes (FP32 fluxes: the
+// combined flux) and, with FP64 fluxes and viscosity, NV - 1 viscous planes
+int alloc_face_fluxes(fvg_ctx* c, const fvg_config& cfg) {
+    const size_t fe = c->fp32 ? sizeof(float) : sizeof(double);
+    int rc = FVG_OK;
+    for (int d = 0; d < c->ND; ++d) {
+        if (!c->Fc[d]) rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)c->NV * c->nfa[d], fe);
+        if (cfg.use_viscous && !c->fp32 && !c->Fv[d]) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(c->NV - 1) * c->nfa[d], fe);
+    }
+    return rc;
 }
 
 }   // namespace
@@ -686,6 +759,26 @@ void fill_kernel_params(const fvg_config& cfg, KP& p) {
         p.mg_fj[d] = magic_for((unsigned)(cfg.ny + (d == 1)));
     }
     p.npad = (long long)p.px * p.py * (cfg.nz + 2 * p.phz);
+    // line segments of the fused passes along y and z: 4m - 1 cells (4m faces: whole steps of the block's 4
+    // wavefronts), the longest of 127, 63, .., 3 that still gives the last axis' pass 1536 blocks (two rounds of
+    // three blocks per CU); a segment of m cells computes m + 1 faces. FVG_LINE_SEGMENT overrides it (tuning).
+    p.nlchunk = (cfg.nx + 63) / 64;
+    p.mg_lchunk = magic_for((unsigned)p.nlchunk);
+    {
+        const int nl = three ? cfg.nz : cfg.ny;
+        const long long lines = (long long)p.nlchunk * (three ? cfg.ny : 1);
+        int seg = 127;
+        while (seg > 3 && lines * ((nl + seg - 1) / seg) < 1536) seg = (seg + 1) / 2 - 1;
+        if (const char* e = std::getenv("FVG_LINE_SEGMENT")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= 4096) seg = v;
+        }
+        p.seg = seg;
+        for (int a = 0; a < 3; ++a) {
+            p.nseg[a] = (n3[a] + seg - 1) / seg;
+            p.mg_nseg[a] = magic_for((unsigned)p.nseg[a]);
+        }
+    }
     p.cfl = cfg.cfl;
     p.Ldt = fvg_dt_length(&cfg);
     p.viscous = visc;
@@ -752,11 +845,10 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
     rc = rc ? rc : checked_alloc(c, &c->W, (size_t)NV * npad, fe);
     if (visc) rc = rc ? rc : checked_alloc(c, &c->G, (size_t)ND * ND * c->nc, fe);
     if (hancock) rc = rc ? rc : checked_alloc(c, &c->HS, (size_t)2 * ND * NV * c->nc);
-    for (int d = 0; d < ND; ++d) {
-        rc = rc ? rc : checked_alloc(c, &c->Fc[d], (size_t)NV * c->nfa[d], fe);
-        if (visc && !c->fp32) rc = rc ? rc : checked_alloc(c, &c->Fv[d], (size_t)(NV - 1) * c->nfa[d], fe);   // fp32: combined with Fc
-    }
     c->quad4 = cfg.flux_quadrature == FVG_QUAD_FOURTH;
+    // the face-flux planes of the separate passes (the default, and the fourth-order quadrature's); the accumulators
+    // of the fused passes are allocated when fvg_set_fused switches to them
+    rc = rc ? rc : alloc_face_fluxes(c, cfg);
     const long long q4_n = std::max({c->nfa[0], c->nfa[1], c->nfa[2]});
     if (c->quad4) {   // the padded conservative state and one axis' face states, flags and point-value fluxes
         rc = rc ? rc : checked_alloc(c, &c->Wc, (size_t)NV * npad);
@@ -814,6 +906,7 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
         p.Fv[d] = c->Fv[d];
         p.nfa[d] = c->nfa[d];
     }
+    p.A = c->A;
     p.st = c->st;
     p.spd = c->spd;
     p.bp.ut = reinterpret_cast<const double*>(reinterpret_cast<const char*>(c->st) + offsetof(DevState, ut));
@@ -900,9 +993,8 @@ int fvg_residual(fvg_ctx* c, double half_dt, double* R, int ncomp, double* s_max
     s.half_dt = hancock ? half_dt : 0.0;
     if ((rc = write_state(c, s))) return rc;
     if ((rc = enqueue_control(c, CTL_BEGIN_STEP, 0))) return rc;
-    if ((rc = enqueue_residual(c, c->U, hancock ? MODE_HANCOCK : plain_mode(c)))) return rc;
+    if ((rc = residual_into(c, c->U, hancock ? MODE_HANCOCK : plain_mode(c), false, c->Rb))) return rc;
     if ((rc = enqueue_control(c, CTL_RESIDUAL_S, 0))) return rc;
-    if ((rc = enqueue_update(c, UPD_RESIDUAL, nullptr, nullptr, c->Rb))) return rc;
     DevState o;
     if ((rc = read_state(c, o))) return rc;
     if (s_max) *s_max = o.s_step;
@@ -1132,6 +1224,12 @@ int fvg_indicator_samples(fvg_ctx* c, int axis, float* X, float* dep, long long 
 
 int fvg_set_fused(fvg_ctx* c, int on) {
     if (!c) return FVG_EINVAL;
+    if (on && !c->quad4 && !c->A) {   // fused passes: the accumulator planes
+        HIPCHK(c, hipSetDevice(c->device));
+        const bool visc = c->cfg.use_viscous != 0;
+        if (int rc = checked_alloc(c, &c->A, (size_t)(visc && !c->fp32 ? 2 * c->NV - 1 : c->NV) * c->nc)) return rc;
+        c->base.A = c->A;
+    }
     c->fused = on != 0;
     if (c->graph) {   // the captured graph holds the old kernel sequence
         hipGraphExecDestroy(c->graph);

@@ -63,6 +63,12 @@ struct KP {
     int rbt, nchunk, slabs;   // slabs: 1 when ny divides by 8 (each XCD takes a y-slab), else 0 (rows in mesh order)
     Magic mg_chunk, mg_slab, mg_ny;
     Magic mg_fi[3], mg_fj[3];   // faces per row and rows per plane of the faces normal to each axis
+    // fused residual passes (k_xacc, k_lacc): accumulator planes A of the partial sums of the divergence (FP64: NV
+    // convective, then NV - 1 viscous with FP64 fluxes and viscosity), and the line segments of the passes along y
+    // and z: seg cells each, nseg[d] segments per line along axis d, blocks of 64 x positions
+    double* A;
+    int seg, nseg[3], nlchunk;   // nlchunk: 64-lane chunks of an x-row
+    Magic mg_nseg[3], mg_lchunk;
     // buffers (plane sets)
     // W, G, Fc and Fv hold doubles in the FP64 mode and floats in the FP32-flux mode (flux_precision)
     const double* U;    // stage input state (always FP64)

@@ -14,20 +14,10 @@ namespace fvg {
     template __global__ void k_update<NV, UPD_RK2_2, T>(KP, const double*, const double*, double*);                 \
     template __global__ void k_update<NV, UPD_RK3_2, T>(KP, const double*, const double*, double*);                 \
     template __global__ void k_update<NV, UPD_RK3_3, T>(KP, const double*, const double*, double*);
-#define FVG_ZUPD(NV, M, T)                                                                                 \
-    template __global__ void k_zupdate<NV, M, UPD_RK2_2, T>(KP, const double*, const double*, double*); \
-    template __global__ void k_zupdate<NV, M, UPD_RK3_2, T>(KP, const double*, const double*, double*); \
-    template __global__ void k_zupdate<NV, M, UPD_RK3_3, T>(KP, const double*, const double*, double*);
 FVG_INST_T(4, float)
 FVG_INST_T(5, float)
 template __global__ void k_faces<5, 2, MODE_PLAIN, float>(KP);
 template __global__ void k_faces<5, 2, MODE_FIRST_ORDER, float>(KP);
-FVG_ZUPD(5, MODE_PLAIN, float)
-FVG_ZUPD(5, MODE_FIRST_ORDER, float)
-FVG_ZUPD(5, MODE_LEARNED, float)
-FVG_ZUPD(4, MODE_PLAIN, float)
-FVG_ZUPD(4, MODE_FIRST_ORDER, float)
-FVG_ZUPD(4, MODE_LEARNED, float)
 #define FVG_LEARNED(T)                                                                \
     template __global__ void k_faces_learned<4, 0, LF_STORE, T>(KP);                  \
     template __global__ void k_faces_learned<4, 1, LF_STORE, T>(KP);                  \
@@ -50,5 +40,32 @@ FVG_ZUPD(4, MODE_LEARNED, float)
     template __global__ void k_samples<5, 1, T>(KP, float*, float*);                  \
     template __global__ void k_samples<5, 2, T>(KP, float*, float*);
 FVG_LEARNED(float)
+// the fused residual passes (k_xacc, k_lacc) of every mode and stage the host launches
+#define FVG_FUSED_MODE(NV, M, USE, T) template __global__ void k_xacc<NV, M, USE, T>(KP);
+#define FVG_YPASS(M, USE, T) template __global__ void k_lacc<5, 1, M, USE, -1, T>(KP, const double*, const double*, double*);
+#define FVG_LAST(NV, M, USE, UPD, T) \
+    template __global__ void k_lacc<NV, (NV == 5 ? 2 : 1), M, USE, UPD, T>(KP, const double*, const double*, double*);
+#define FVG_LAST_ALL(NV, M, T)                  \
+    FVG_LAST(NV, M, 0, UPD_RESIDUAL, T)         \
+    FVG_LAST(NV, M, 0, UPD_RK2_2, T)            \
+    FVG_LAST(NV, M, 0, UPD_RK3_2, T)            \
+    FVG_LAST(NV, M, 0, UPD_RK3_3, T)
+#define FVG_FUSED(NV, T)                                      \
+    FVG_FUSED_MODE(NV, MODE_PLAIN, 0, T)                      \
+    FVG_FUSED_MODE(NV, MODE_FIRST_ORDER, 0, T)                \
+    FVG_FUSED_MODE(NV, MODE_LEARNED, LF_STORE, T)             \
+    FVG_FUSED_MODE(NV, MODE_LEARNED, LF_KEPT, T)              \
+    FVG_LAST_ALL(NV, MODE_PLAIN, T)                           \
+    FVG_LAST_ALL(NV, MODE_FIRST_ORDER, T)                     \
+    FVG_LAST(NV, MODE_LEARNED, LF_STORE, UPD_RESIDUAL, T)     \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK2_2, T)         \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK3_2, T)         \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK3_3, T)
+FVG_FUSED(4, float)
+FVG_FUSED(5, float)
+FVG_YPASS(MODE_PLAIN, 0, float)
+FVG_YPASS(MODE_FIRST_ORDER, 0, float)
+FVG_YPASS(MODE_LEARNED, LF_STORE, float)
+FVG_YPASS(MODE_LEARNED, LF_KEPT, float)
 
 }   // namespace fvg

@@ -14,10 +14,6 @@ namespace fvg {
     template __global__ void k_update<NV, UPD_RK2_2, T>(KP, const double*, const double*, double*);                 \
     template __global__ void k_update<NV, UPD_RK3_2, T>(KP, const double*, const double*, double*);                 \
     template __global__ void k_update<NV, UPD_RK3_3, T>(KP, const double*, const double*, double*);
-#define FVG_ZUPD(NV, M, T)                                                                                 \
-    template __global__ void k_zupdate<NV, M, UPD_RK2_2, T>(KP, const double*, const double*, double*); \
-    template __global__ void k_zupdate<NV, M, UPD_RK3_2, T>(KP, const double*, const double*, double*); \
-    template __global__ void k_zupdate<NV, M, UPD_RK3_3, T>(KP, const double*, const double*, double*);
 FVG_INST_T(4, double)
 FVG_INST_T(5, double)
 template __global__ void k_faces<5, 2, MODE_PLAIN, double>(KP);
@@ -27,12 +23,6 @@ template __global__ void k_faces<4, 1, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<5, 0, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<5, 1, MODE_HANCOCK, double>(KP);
 template __global__ void k_faces<5, 2, MODE_HANCOCK, double>(KP);
-FVG_ZUPD(5, MODE_PLAIN, double)
-FVG_ZUPD(5, MODE_FIRST_ORDER, double)
-FVG_ZUPD(4, MODE_PLAIN, double)
-FVG_ZUPD(4, MODE_FIRST_ORDER, double)
-FVG_ZUPD(5, MODE_LEARNED, double)
-FVG_ZUPD(4, MODE_LEARNED, double)
 #define FVG_LEARNED(T)                                                                \
     template __global__ void k_faces_learned<4, 0, LF_STORE, T>(KP);                  \
     template __global__ void k_faces_learned<4, 1, LF_STORE, T>(KP);                  \
@@ -55,5 +45,39 @@ FVG_ZUPD(4, MODE_LEARNED, double)
     template __global__ void k_samples<5, 1, T>(KP, float*, float*);                  \
     template __global__ void k_samples<5, 2, T>(KP, float*, float*);
 FVG_LEARNED(double)
+// the fused residual passes (k_xacc, k_lacc) of every mode and stage the host launches
+#define FVG_FUSED_MODE(NV, M, USE, T) template __global__ void k_xacc<NV, M, USE, T>(KP);
+#define FVG_YPASS(M, USE, T) template __global__ void k_lacc<5, 1, M, USE, -1, T>(KP, const double*, const double*, double*);
+#define FVG_LAST(NV, M, USE, UPD, T) \
+    template __global__ void k_lacc<NV, (NV == 5 ? 2 : 1), M, USE, UPD, T>(KP, const double*, const double*, double*);
+#define FVG_LAST_ALL(NV, M, T)                  \
+    FVG_LAST(NV, M, 0, UPD_RESIDUAL, T)         \
+    FVG_LAST(NV, M, 0, UPD_RK2_2, T)            \
+    FVG_LAST(NV, M, 0, UPD_RK3_2, T)            \
+    FVG_LAST(NV, M, 0, UPD_RK3_3, T)
+#define FVG_FUSED(NV, T)                                      \
+    FVG_FUSED_MODE(NV, MODE_PLAIN, 0, T)                      \
+    FVG_FUSED_MODE(NV, MODE_FIRST_ORDER, 0, T)                \
+    FVG_FUSED_MODE(NV, MODE_LEARNED, LF_STORE, T)             \
+    FVG_FUSED_MODE(NV, MODE_LEARNED, LF_KEPT, T)              \
+    FVG_LAST_ALL(NV, MODE_PLAIN, T)                           \
+    FVG_LAST_ALL(NV, MODE_FIRST_ORDER, T)                     \
+    FVG_LAST(NV, MODE_LEARNED, LF_STORE, UPD_RESIDUAL, T)     \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK2_2, T)         \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK3_2, T)         \
+    FVG_LAST(NV, MODE_LEARNED, LF_KEPT, UPD_RK3_3, T)
+FVG_FUSED(4, double)
+FVG_FUSED(5, double)
+FVG_YPASS(MODE_PLAIN, 0, double)
+FVG_YPASS(MODE_FIRST_ORDER, 0, double)
+FVG_YPASS(MODE_LEARNED, LF_STORE, double)
+FVG_YPASS(MODE_LEARNED, LF_KEPT, double)
+FVG_FUSED_MODE(4, MODE_HANCOCK, 0, double)
+FVG_FUSED_MODE(5, MODE_HANCOCK, 0, double)
+FVG_YPASS(MODE_HANCOCK, 0, double)
+FVG_LAST(4, MODE_HANCOCK, 0, UPD_RESIDUAL, double)
+FVG_LAST(5, MODE_HANCOCK, 0, UPD_RESIDUAL, double)
+template __global__ void k_axpy<4>(KP, const double*, const double*, double*);
+template __global__ void k_axpy<5>(KP, const double*, const double*, double*);
 
 }   // namespace fvg

@@ -1171,35 +1171,42 @@ __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// k_zupdate: the faces of the last axis (z in 3-D, y in 2-D) and the cell update in one pass, for the stages
-// whose dt is already known (stage 2 of SSP-RK2, stages 2 and 3 of SSP-RK3). A block of 1024 threads takes 64
-// x-positions by 16 faces along the last axis: every thread computes one face into LDS, then 15 of the 16 levels
-// update their cell from the face fluxes of the earlier passes and the two fluxes in LDS, in k_update's order. The
-// tile's top face is also the bottom face of the tile above (the same bits, counted twice in the maximum, which
-// does not change it). The output never aliases the stage input, so no thread reads a cell another thread has
-// already updated.
+// Fused residual passes (the default kernel sequence): the face fluxes never go to HBM. Every face is computed
+// once (up to one face per line segment or wavefront) and its difference with the next face of the same axis is
+// added at once to the cell between them, in the reference's order of the divergence (Simulation.cpp:1022-1037,
+// 1293-1310; viscous balance Physics.cpp:196-204, 746-752):
+//   FP64: a = ((0 - (fr Ax - fl Ax)) - (ft Ay - fb Ay)) - (fk Az - ff Az), R = a / V;
+//         v = ((0 + (vr - vl)) + (vt - vb)) + (vk - vf), R = R + v / V (R[0] + 0);
+//   FP32 fluxes (combined F A - v): d = ((fr - fl) + (ft - fb)) + (fk - ff) in double, R = -d / V.
+// The partial sums after the first axes live in the accumulator planes A (FP64: NV convective, NV - 1 viscous);
+// the last axis' pass finishes R and either stores it (UPD_RESIDUAL) or applies the stage combination at once
+// (UPD_RK2_2, UPD_RK3_2, UPD_RK3_3: dt is known). The stage output never aliases the stage input, so no face reads
+// a cell another thread has already updated.
+//   x: k_xacc, one face per lane in face order; a wavefront holds 64 consecutive faces and updates the 63 cells
+//      between them (neighbour face by lane shuffle), so wavefronts overlap by one face;
+//   y (3-D), then the last axis: k_lacc, one lane per line segment of `seg` cells along the axis (64 lanes on
+//      consecutive x), looping over its seg + 1 faces and keeping the last face's fluxes.
 // ------------------------------------------------------------------------------------------------------------
-constexpr int ZT_I = 64, ZT_K = 16;   // k_zupdate tile: 64 faces along x by 16 along the last axis (15 cells updated)
 
-// face (i, j, k) of the last axis D (z in 3-D, y in 2-D) with its coordinate kf along D
-template <int NV, int MODE, typename T>
-__device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j, int k, int kf, T* Fc, T* Fv, double& local) {
-    constexpr int D = NV == 5 ? 2 : 1;
+// The fluxes of face (i, j, k) normal to D (t its coordinate along D) as the separate face pass stores them
+// (store_face): FP64 the convective flux Fc and, with viscosity, the viscous flux Fv; FP32 the combined flux in Fc.
+// Every lane of a wavefront calls it in the learned mode (the network runs on the matrix cores); lanes with
+// active = false only take part in the network. The wave speed of the face goes into local.
+template <int NV, int D, int MODE, int USE, typename T>
+__device__ __forceinline__ void face_all(const KP& p, bool active, long long f, int i, int j, int k, int t, T* Fc, T* Fv, double& local) {
     const int n = ncell(p, D);
     T sm = T(0);
     if constexpr (MODE == MODE_LEARNED) {
-        // stages 2-3: the decisions the network made on the step's first stage, as in the separate face passes
-        const long long f = D == 2 ? ((long long)k * p.ny + j) * p.nx + i : (long long)j * p.nx + i;
-        learned_face<NV, D, LF_KEPT, T>(p, active, f, i, j, k, kf, n, Fc, sm);
+        learned_face<NV, D, USE, T>(p, active, f, i, j, k, t, n, Fc, sm);
     } else {
         if (!active) return;
         T UL[NV], UR[NV];
-        face_states<NV, D, MODE, T>(p, i, j, k, kf, n, UL, UR);
+        face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
         face_flux<NV>(p.rs, UL, UR, D, Fc, sm);
     }
     if (!active) return;
     if (double(sm) > local) local = double(sm);
-    if (p.viscous) viscous_face<NV, D, T>(p, i, j, k, kf, n, Fv);
+    if (p.viscous) viscous_face<NV, D, T>(p, i, j, k, t, n, Fv);
     if constexpr (std::is_same<T, float>::value) {   // the combined flux of store_face
         const float A = float(area_of(p, D));
 #pragma unroll
@@ -1210,91 +1217,187 @@ __device__ __forceinline__ void last_face(const KP& p, bool active, int i, int j
     }
 }
 
-template <int NV, int MODE, int UPD, typename T>
-__global__ __launch_bounds__(1024) void k_zupdate(KP p, const double* __restrict__ U0, const double* __restrict__ Ut,
-                                                  double* __restrict__ out) {
-    constexpr int D = NV == 5 ? 2 : 1;
-    __shared__ T sF[ZT_K][NV][ZT_I];
-    __shared__ T sV[ZT_K][NV - 1][ZT_I];
-    if (p.st->done) return;   // uniform
-    const int li = threadIdx.x % ZT_I, lk = threadIdx.x / ZT_I;
-    const int nit = (p.nx + ZT_I - 1) / ZT_I;
-    const long long b = xcd_block();
-    const int it = (int)(b % nit);
-    const long long r = b / nit;
-    // 3-D: tiles (i, j, z); 2-D: tiles (i, y) of the single plane
-    const int j = D == 2 ? (int)(r % p.ny) : 0, kt = D == 2 ? (int)(r / p.ny) : (int)r;
-    const int n = ncell(p, D);
-    const int i = it * ZT_I + li, kf = kt * (ZT_K - 1) + lk;   // this thread's face along D
-    const bool af = i < p.nx && kf <= n;
-    const int fj = D == 2 ? j : kf, fk = D == 2 ? kf : 0;     // the face's (j, k)
+template <int NV, int MODE, int USE, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xacc(KP p) {
+    if (p.st->done) return;   // uniform: the whole grid leaves
+    const long long nf = (long long)(p.nx + 1) * p.ny * p.nz;
+    const int lane = threadIdx.x & 63;
+    const long long f = (xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 63 + lane;
+    const bool active = f < nf;
+    int i = 0, j = 0, k = 0, t = 0;
+    if (active) face_coords<NV, 0>(p, f, i, j, k, t);
+    T Fc[NV], Fv[NV - 1];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Fc[q] = T(0);
+#pragma unroll
+    for (int q = 0; q < NV - 1; ++q) Fv[q] = T(0);
     double local = 0.0;
-    T Ft[NV], Vt[NV - 1];
+    face_all<NV, 0, MODE, USE, T>(p, active, f, i, j, k, t, Fc, Fv, local);
+    // the next face along x is the next lane's (every lane takes part in the exchange)
+    T Gc[NV], Gv[NV - 1];
 #pragma unroll
-    for (int q = 0; q < NV - 1; ++q) Vt[q] = T(0);
-    last_face<NV, MODE, T>(p, af, i, fj, fk, kf, Ft, Vt, local);   // a wavefront holds one level and 64 x-positions
-    if (af) {
+    for (int q = 0; q < NV; ++q) Gc[q] = __shfl_down(Fc[q], 1, 64);
+    constexpr bool VISC64 = std::is_same<T, double>::value;
+    if (VISC64 && p.viscous)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) sF[lk][q][li] = Ft[q];
+        for (int q = 0; q < NV - 1; ++q) Gv[q] = __shfl_down(Fv[q], 1, 64);
+    if (active && lane < 63 && i < p.nx) {   // the cell right of this face (its right face is face f + 1)
+        const long long c = cidx(p, i, j, k);
+        double* A = p.A;
+        if constexpr (VISC64) {
 #pragma unroll
-        for (int q = 0; q < NV - 1; ++q) sV[lk][q][li] = Vt[q];
-    }
-    __syncthreads();
-    if (lk < ZT_K - 1 && af && kf < n) {   // the cell between faces lk and lk + 1 of the tile
-        const int cj = fj, ck = fk;
-        const long long c = cidx(p, i, cj, ck);
-        const long long fx = ((long long)ck * p.ny + cj) * (p.nx + 1) + i;
-        const long long fy = ((long long)ck * (p.ny + 1) + cj) * p.nx + i;
-        const long long fyt = fy + p.nx;
-        double R[NV];
-        if constexpr (std::is_same<T, float>::value) {
+            for (int q = 0; q < NV; ++q) A[q * p.nc + c] = 0.0 - (double(Gc[q]) * p.Ax - double(Fc[q]) * p.Ax);
+            if (p.viscous)
 #pragma unroll
-            for (int q = 0; q < NV; ++q) {
-                const float* Fx = (const float*)p.Fc[0] + q * p.nfa[0];
-                double d = double(Fx[fx + 1]) - double(Fx[fx]);
-                if constexpr (D == 2) {
-                    const float* Fy = (const float*)p.Fc[1] + q * p.nfa[1];
-                    d = d + (double(Fy[fyt]) - double(Fy[fy]));
-                }
-                d += double(sF[lk + 1][q][li]) - double(sF[lk][q][li]);
-                R[q] = -d * p.inv_vol;
-            }
+                for (int q = 0; q < NV - 1; ++q) A[(NV + q) * p.nc + c] = 0.0 + (double(Gv[q]) - double(Fv[q]));
         } else {
 #pragma unroll
-            for (int q = 0; q < NV; ++q) {
-                const double* Fx = (const double*)p.Fc[0] + q * p.nfa[0];
-                const double fl = Fx[fx], fr = Fx[fx + 1];
-                const double fa = sF[lk][q][li], fz = sF[lk + 1][q][li];
-                if constexpr (D == 2) {
-                    const double* Fy = (const double*)p.Fc[1] + q * p.nfa[1];
-                    const double fb = Fy[fy], ft = Fy[fyt];
-                    R[q] = (((0.0 - (fr * p.Ax - fl * p.Ax)) - (ft * p.Ay - fb * p.Ay)) - (fz * p.Az - fa * p.Az)) * p.inv_vol;
-                } else {
-                    R[q] = ((0.0 - (fr * p.Ax - fl * p.Ax)) - (fz * p.Ay - fa * p.Ay)) * p.inv_vol;
-                }
-            }
-            if (p.viscous) {
-                R[0] = R[0] + 0.0;
+            for (int q = 0; q < NV; ++q) A[q * p.nc + c] = double(Gc[q]) - double(Fc[q]);
+        }
+    }
+    block_max_slot(p, local, SPD_CONV);
+}
+
+// y faces (3-D, D = 1, UPD = -1: add to A) or the last axis (z in 3-D, y in 2-D; UPD: finish R, then store it or
+// combine). A block is 4 wavefronts on 64 consecutive x positions of one line segment of seg cells along D; the
+// wavefronts take 4 consecutive faces of the segment at a time (wavefront w face tb + w), so that they share their
+// stencil rows in the caches, and step along the segment together. The face below a wavefront's cell is the
+// previous wavefront's (wavefront 0: the last wavefront's of the previous step), passed through LDS between two
+// barriers. Block b of the grid: 64-lane chunk along x, segment, and the other transverse coordinate o (z for D = 1,
+// y for D = 2), in the order (o, segment, chunk) over the XCDs' contiguous ranges.
+template <int NV, int D, int MODE, int USE, int UPD, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_lacc(KP p, const double* __restrict__ U0,
+                                                                                                        const double* __restrict__ Ut,
+                                                                                                        double* __restrict__ out) {
+    if (p.st->done) return;   // uniform
+    const unsigned b = (unsigned)xcd_block();
+    const unsigned r = mdiv(b, p.mg_lchunk), chunk = b - r * p.nlchunk;
+    const unsigned o = mdiv(r, p.mg_nseg[D]), sidx = r - o * p.nseg[D];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = chunk * 64 + lane;
+    const bool lane_ok = i < p.nx;
+    const int n = ncell(p, D);
+    const int t0 = (int)sidx * p.seg, t1 = t0 + p.seg < n ? t0 + p.seg : n;   // cells t0 .. t1 - 1, faces t0 .. t1
+    constexpr bool VISC64 = std::is_same<T, double>::value;
+    constexpr int NA = VISC64 ? 2 * NV - 1 : NV;
+    const double area = D == 1 ? p.Ay : p.Az;
+    double local = 0.0;
+    __shared__ T sF[4][NA][64];   // the fluxes of each wavefront's face of the step
+    for (int tb = t0; tb <= t1; tb += 4) {   // uniform over the block
+        // the parameters (read in place in the kernel-argument segment, where p is the first argument) and the
+        // coordinates pass through an empty asm each step, so that the compiler re-reads them instead of hoisting
+        // the face code's address arithmetic out of the loop (which spills)
+        typedef const __attribute__((address_space(4))) KP* KPconst;
+        KPconst pp = (KPconst)__builtin_amdgcn_kernarg_segment_ptr();
+        int ii = i;
+        asm volatile("" : "+s"(pp));
+        asm volatile("" : "+v"(ii));
+        const KP& q = *(const KP*)pp;
+        const int t = tb + w;   // this wavefront's face
+        const bool face = lane_ok && t <= t1;
+        const int jj = D == 1 ? t : (int)o, kk = D == 2 ? t : (D == 1 ? (int)o : 0);
+        const long long f = D == 1 ? ((long long)kk * (p.ny + 1) + jj) * p.nx + i : ((long long)kk * p.ny + jj) * p.nx + i;
+        // the cell this wavefront finishes (between face t - 1 below and face t above): its partial sums and, for a
+        // stage combination, its stage states are loaded before the face, so that their latency hides behind it
+        const bool cell = face && t > t0;
+        const long long c = cidx(p, i, D == 1 ? t - 1 : jj, D == 2 ? t - 1 : kk);
+        const bool visc_acc = VISC64 && p.viscous;
+        double a[NA], u0[NV], ut[NV];
+        if (cell) {
 #pragma unroll
-                for (int q = 1; q < NV; ++q) {
-                    const double* Fx = (const double*)p.Fv[0] + (q - 1) * p.nfa[0];
-                    const double fl = Fx[fx], fr = Fx[fx + 1];
-                    const double va = sV[lk][q - 1][li], vz = sV[lk + 1][q - 1][li];
-                    double v;
-                    if constexpr (D == 2) {
-                        const double* Fy = (const double*)p.Fv[1] + (q - 1) * p.nfa[1];
-                        const double fb = Fy[fy], ft = Fy[fyt];
-                        v = (((0.0 + (fr - fl)) + (ft - fb)) + (vz - va)) * p.inv_vol;
-                    } else {
-                        v = ((0.0 + (fr - fl)) + (vz - va)) * p.inv_vol;
+            for (int m = 0; m < NA; ++m) a[m] = (m < NV || visc_acc) ? p.A[m * p.nc + c] : 0.0;
+            if constexpr (UPD == UPD_RK2_2 || UPD == UPD_RK3_2 || UPD == UPD_RK3_3)
+#pragma unroll
+                for (int m = 0; m < NV; ++m) {
+                    u0[m] = U0[m * p.nc + c];
+                    ut[m] = Ut[m * p.nc + c];
+                }
+        }
+        T Fc[NV], Fv[NV - 1];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) Fc[m] = T(0);
+#pragma unroll
+        for (int m = 0; m < NV - 1; ++m) Fv[m] = T(0);
+        face_all<NV, D, MODE, USE, T>(q, face, f, ii, jj, kk, t, Fc, Fv, local);   // every lane of the wavefront
+        if (w < 3) {
+#pragma unroll
+            for (int m = 0; m < NV; ++m) sF[w][m][lane] = Fc[m];
+            if constexpr (VISC64)
+#pragma unroll
+                for (int m = 0; m < NV - 1; ++m) sF[w][NV + m][lane] = Fv[m];
+        }
+        __syncthreads();
+        if (cell) {
+            const int pw = w > 0 ? w - 1 : 3;   // the wavefront that holds face t - 1
+            double* A = p.A;
+            if constexpr (UPD < 0) {   // y faces of a 3-D mesh: add to the partial sums
+                if constexpr (VISC64) {
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) A[m * p.nc + c] = a[m] - (double(Fc[m]) * area - double(sF[pw][m][lane]) * area);
+                    if (p.viscous)
+#pragma unroll
+                        for (int m = 0; m < NV - 1; ++m) A[(NV + m) * p.nc + c] = a[NV + m] + (double(Fv[m]) - double(sF[pw][NV + m][lane]));
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) A[m * p.nc + c] = a[m] + (double(Fc[m]) - double(sF[pw][m][lane]));
+                }
+            } else {   // the last axis: R of the cell, then the stage output
+                double R[NV];
+                if constexpr (VISC64) {
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) R[m] = (a[m] - (double(Fc[m]) * area - double(sF[pw][m][lane]) * area)) * p.inv_vol;
+                    if (p.viscous) {
+                        R[0] = R[0] + 0.0;   // the zero viscous flux of the density slot
+#pragma unroll
+                        for (int m = 1; m < NV; ++m) {
+                            const double v = (a[NV + m - 1] + (double(Fv[m - 1]) - double(sF[pw][NV + m - 1][lane]))) * p.inv_vol;
+                            R[m] = R[m] + v;
+                        }
                     }
-                    R[q] = R[q] + v;
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) R[m] = -(a[m] + (double(Fc[m]) - double(sF[pw][m][lane]))) * p.inv_vol;
+                }
+                if constexpr (UPD == UPD_RESIDUAL) {
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) out[m * p.nc + c] = R[m];
+                } else {
+                    const double dt = p.st->dt;
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) {
+                        double v;
+                        if constexpr (UPD == UPD_RK2_2) v = 0.5 * u0[m] + 0.5 * (ut[m] + dt * R[m]);
+                        else if constexpr (UPD == UPD_RK3_2) v = 0.75 * u0[m] + 0.25 * (ut[m] + dt * R[m]);
+                        else v = (1.0 / 3.0) * u0[m] + (2.0 / 3.0) * (ut[m] + dt * R[m]);
+                        out[m * p.nc + c] = v;
+                    }
                 }
             }
         }
-        combine<NV, UPD>(p, c, R, U0, Ut, out);
+        __syncthreads();
+        if (w == 3) {   // the face below wavefront 0's cell of the next step
+#pragma unroll
+            for (int m = 0; m < NV; ++m) sF[3][m][lane] = Fc[m];
+            if constexpr (VISC64)
+#pragma unroll
+                for (int m = 0; m < NV - 1; ++m) sF[3][NV + m][lane] = Fv[m];
+        }
     }
     block_max_slot(p, local, SPD_CONV);
+}
+
+// The first-stage update out = U0 + dt R of a residual the fused passes stored (TimeScheme.cpp:27-59), one cell per
+// thread; out may be U0.
+template <int NV>
+__global__ __launch_bounds__(256) void k_axpy(KP p, const double* U0, const double* __restrict__ R, double* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nc || p.st->done) return;
+    const double dt = p.st->dt;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const long long x = q * p.nc + c;
+        out[x] = U0[x] + dt * R[x];
+    }
 }
 
 }   // namespace fvg

@@ -228,9 +228,12 @@ int fvg_set_timing(fvg_ctx* ctx, int on);
  * context that holds four steps of the scheme; 0 enqueues every kernel. Results are the same bits either way;
  * with event timing on, kernels are always enqueued one by one. */
 int fvg_set_graphs(fvg_ctx* ctx, int on);
-/* Kernel sequence of the Runge-Kutta stages after the first (no counterpart in the reference): 1 computes the
- * faces of the last axis (z in 3-D, y in 2-D) and updates the cells in one pass (1024-thread tiles); 0 (default)
- * keeps separate face and update passes. Same bits. */
+/* Kernel sequence of the residual (no counterpart in the reference): 1, fused passes, in which no face flux goes to
+ * device memory: the x faces' differences go straight into per-cell partial sums, the y faces' are added to them,
+ * and the last axis' pass finishes the residual and applies the stage combination; 0 (default), separate face
+ * passes storing the fluxes and one update pass. The same bits either way; on MI355X the separate passes measured
+ * as fast for FP64 fluxes and faster for FP32 fluxes and the learned indicator. The fourth-order quadrature always
+ * takes its own passes. */
 int fvg_set_fused(fvg_ctx* ctx, int on);
 
 /* Known-answer entry: the Riemann flux of n faces with host AoS conservative states (5 doubles per face),
