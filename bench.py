@@ -362,16 +362,45 @@ def side_workload(name: str) -> dict:
                             "(graph replay)", "value": round(65536 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                 "ms_per_step": round(el / k * 1e3, 4), "steps": k, "cpu_baseline": cpu_baseline(256, 10.0, workload="vortex")}
     if name == "kh":
-        n = 2048
-        with fvg.Solver(fvg.config_from_keys(kh_keys(n))) as s:
-            s.upload(kelvin_helmholtz(n))
-            s.advance(0.0, 1e9, max_steps=2)
-            t0 = time.perf_counter()
-            _, k, _ = s.advance(0.0, 1e9, max_steps=30)
-            el = time.perf_counter() - t0
-        return {"workload": "Kelvin-Helmholtz 2048^2, WENO5-Z + HLLC + SSP-RK3, CFL 0.5, periodic, 30 steps",
-                "value": round(n * n * 3 * k / el / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
-                "ms_per_step": round(el / k * 1e3, 4), "steps": k, "cpu_baseline": cpu_baseline(n, 15.0, workload="kh")}
+        # plain FP64 WENO5-Z (the reference's bits), then the learned indicator with FP64 and with FP32 fluxes and plain
+        # FP32: each from the initial field, 2 warm-up steps and 30 timed steps at a fixed dt (9e-5, below the CFL step
+        # of 1.0e-4) so that every mode reaches the same time; the density's L1 / Linf error against the FP64 run and
+        # the audit counts (every 16 steps) are reported. The layer's density jumps are troubled faces from the start.
+        from fenice_amd.train_indicator import DEFAULT_OUT, load
+        n, dt, steps = 2048, 9e-5, 30
+        U0 = kelvin_helmholtz(n)
+        params, thr, _, _ = load(DEFAULT_OUT)
+        modes, ref, out = [("fp64", False), ("fp64+learned", True), ("fp32", False), ("fp32+learned", True)], None, {}
+        for name_m, learned in modes:
+            keys = dict(kh_keys(n), flux_precision=name_m[:4])
+            if learned:
+                keys.update(indicator="learned", indicator_audit="16")
+            with fvg.Solver(fvg.config_from_keys(keys)) as s:
+                if learned:
+                    s.set_indicator(params, thr)
+                s.upload(U0)
+                t, _, _ = s.advance(0.0, 1e9, max_steps=2, dt_max=dt)
+                t0 = time.perf_counter()
+                t, k, _ = s.advance(t, 1e9, max_steps=steps, dt_max=dt)
+                el = time.perf_counter() - t0
+                rho = s.download()[:, 0]
+                st = s.indicator_stats() if learned else None
+            m = {"value": round(n * n * 3 * k / el / 1e6, 1), "ms_per_step": round(el / k * 1e3, 4), "steps": k, "t": t}
+            if ref is None:
+                ref = rho
+            else:
+                d = np.abs(rho - ref)
+                m["error_vs_fp64"] = {"l1_rel": float(d.sum() / np.abs(ref).sum()), "linf_rel": float(d.max() / np.abs(ref).max())}
+            if st is not None:
+                _, total, off, audits = st
+                m["indicator"] = {"audits": audits, "switched_off": off, "tp_fp_fn_tn_all_audits": total,
+                                  "troubled_fraction": (total[0] + total[2]) / max(1, sum(total)),
+                                  "missed_troubled_fraction": total[2] / max(1, total[0] + total[2])}
+            out[name_m] = m
+        r = out.pop("fp64")
+        return {"workload": f"Kelvin-Helmholtz 2048^2, WENO5-Z + HLLC + SSP-RK3, periodic, {steps} steps at dt {dt}",
+                "value": r["value"], "unit": "Mcell-updates/s per RK stage", "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+                "modes": out, "cpu_baseline": cpu_baseline(n, 15.0, workload="kh")}
     if name == "tgv512":
         # the 512^3 Taylor-Green vortex with FP32 fluxes and the learned indicator beside the plain FP64 path: each mode
         # is timed on 8 steps after 1 warm-up step, then restarted from the initial field for 300 steps at a fixed dt

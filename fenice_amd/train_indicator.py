@@ -1,8 +1,9 @@
 """Trains the learned troubled-face indicator on the device and writes its weights.
 
 Training data: the faces of FP32 WENO5-Z runs of several flows (3-D Taylor-Green at two Mach numbers and two
-meshes, smooth random Fourier fields in 2-D and 3-D, a 2-D shock interaction, the Kelvin-Helmholtz shear layer,
-the isentropic vortex, a Sod tube), sampled at several times. Each
+meshes, at Mach 0.1 and 64^3 through its transition to turbulence, smooth random Fourier fields in 2-D and 3-D, a
+2-D shock interaction, the Kelvin-Helmholtz shear layer through its roll-up, the isentropic vortex, a Sod tube),
+sampled at several times. Each
 face carries its 15 features and its label (WENO5-Z weight departure above `label`, the same test the audit makes).
 Smooth faces, the large majority, are subsampled; faces whose label value lies within a factor 2 of the threshold
 are left out of training (they are neither clearly smooth nor clearly troubled); troubled faces are weighted
@@ -92,14 +93,15 @@ def flows():
     out = []
     for mach in (0.1, 0.6):
         k = dict(taylor_green_keys(64, "fp32"))
-        out.append((f"tgv64-mach{mach}", k, taylor_green(64, mach=mach), (0, 30, 90, 200)))
+        # Mach 0.1 through the transition to turbulence (t = 2.7, 4.5, 7.2 at steps 600, 1000, 1600)
+        out.append((f"tgv64-mach{mach}", k, taylor_green(64, mach=mach), (0, 30, 90, 200) + ((600, 1000, 1600) if mach == 0.1 else ())))
     out.append(("tgv128-mach0.1", dict(taylor_green_keys(128, "fp32")), taylor_green(128, mach=0.1), (0, 40)))
     per3 = dict(per2, bc_zmin="periodic", bc_zmax="periodic")
     for seed, (n, nd) in enumerate(((96, 2), (48, 3), (256, 2))):
         keys = dict(base, nx=str(n), ny=str(n), nz=str(n if nd == 3 else 1), Lx=TWO_PI, Ly=TWO_PI, Lz=TWO_PI,
                     **(per3 if nd == 3 else per2))
         out.append((f"modes{n}-{nd}d", keys, _modes(n, nd, seed), (0, 30)))
-    out.append(("kh256", dict(kh_keys(256), flux_precision="fp32"), kelvin_helmholtz(256), (0, 150, 400)))
+    out.append(("kh256", dict(kh_keys(256), flux_precision="fp32"), kelvin_helmholtz(256), (0, 150, 400, 700, 1000)))
     out.append(("vortex128", dict(base, nx="128", ny="128", Lx="10.0", Ly="10.0", **per2), _vortex(128), (0, 100)))
     out.append(("riemann2d-192", dict(base, nx="192", ny="192", Lx="1.0", Ly="1.0", bc_xmin="transmissive",
                                       bc_xmax="transmissive", bc_ymin="transmissive", bc_ymax="transmissive"),
