@@ -933,14 +933,21 @@ template <int NV, int D, int USE, typename T>
 __device__ __forceinline__ void learned_face(const KP& p, bool active, long long f, int i, int j, int k, int t, int n, T* F,
                                              T& sm) {
     const bool upgrade = active && face_upgrade<NV, D>(p, t);
+    // once the audit has switched the indicator off every face takes WENO5-Z and the network is not run (the switch
+    // changes only between steps; read once per wavefront, so the branch is uniform)
+    const bool off = __builtin_amdgcn_readfirstlane(p.ind->off) != 0;
     T c[6][NV];
     bool troubled = true;
     if constexpr (USE == LF_KEPT) {
         if (upgrade) {
 #pragma unroll
             for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
-            troubled = kept_of<NV, D>(p)[f] != 0;
+            troubled = off || kept_of<NV, D>(p)[f] != 0;
         }
+    } else if (off) {   // LF_STORE, indicator off
+        if (upgrade)
+#pragma unroll
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
     } else {   // LF_STORE
         float x[16];
 #pragma unroll
@@ -959,7 +966,7 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
     const int il = D == 0 ? t - 1 : i, jl = D == 1 ? t - 1 : j, kl = D == 2 ? t - 1 : k;
     T UL[NV], UR[NV];
     if (upgrade) {
-        const bool smooth = !troubled && !p.ind->off;
+        const bool smooth = !troubled && !off;
         T wl[NV], wr[NV];
         const T we = T(p.weno_eps);
         if (smooth) {
