@@ -344,12 +344,28 @@ def side_workload(name: str) -> dict:
             block, _ = e.tune(0.0075, 400000)
             r = e.run(0.15, 400000)
         stages = 2.0 * float((n.astype(np.float64) * r.steps).sum())
+        # beside it, 2-D instances: 256 isentropic vortices on square periodic meshes of 32..256 cells a side, MUSCL
+        # (Van Leer) + HLLC + SSP-RK2, CFL 0.5, to t = 1 (the vortex moves by one unit of its 10-wide box)
+        from fenice_amd.workloads import vortex_ensemble, vortex_ensemble_keys
+        nv, Uv = vortex_ensemble()
+        with fvg.Ensemble(fvg.config_from_keys(vortex_ensemble_keys()), nv, ny=nv) as e:
+            e.upload(Uv)
+            vblock, _ = e.tune(0.05, 100000)
+            rv = e.run(1.0, 100000)
+        vstages = 2.0 * float(((nv.astype(np.float64) ** 2) * rv.steps).sum())
         return {"workload": "1024 1-D Riemann problems, 128..8192 cells (4,243,015), MUSCL MC + HLLC + SSP-RK2, CFL 0.5, "
                             "to t = 0.15", "block": block, "kernel_ms": round(r.ms, 2),
                 "value": round(stages / (r.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                 "instances_done": int((r.t >= 0.15).sum()), "diverged": int(r.diverged.sum()),
                 "steps_max": int(r.steps.max()), "steps_total": int(r.steps.sum()),
-                "cpu_baseline": cpu_baseline(16, 20, workload="ensemble")}
+                "cpu_baseline": cpu_baseline(16, 20, workload="ensemble"),
+                "vortex_instances": {"workload": f"256 isentropic vortices, periodic [0, 10]^2, 32..256 cells a side "
+                                                 f"({int((nv.astype(np.int64) ** 2).sum()):,} cells), MUSCL Van Leer + HLLC + "
+                                                 "SSP-RK2, CFL 0.5, to t = 1",
+                                     "block": vblock, "kernel_ms": round(rv.ms, 2),
+                                     "value": round(vstages / (rv.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
+                                     "instances_done": int((rv.t >= 1.0).sum()), "diverged": int(rv.diverged.sum()),
+                                     "steps_max": int(rv.steps.max()), "steps_total": int(rv.steps.sum())}}
     if name == "vortex":
         from fenice_amd.workloads import vortex_keys, vortex_state
         with fvg.Solver(fvg.config_from_keys(vortex_keys(256, cfl="0.7"))) as s:

@@ -1,4 +1,5 @@
-// An ensemble of independent 1-D runs in one persistent kernel (include/fvg.h, fvg_batch_*).
+// An ensemble of independent 1-D runs, or of 2-D runs (fvg_batch_create_2d; see k_ensemble2), in one persistent
+// kernel (include/fvg.h, fvg_batch_*).
 //
 // Instance i is the reference's 2-D run on an N_i x 1 mesh (periodic y, one row): nx = N_i, ny = 1, the scheme
 // and the x boundaries of the shared configuration. One workgroup owns one instance for its whole run: it loops
@@ -159,14 +160,17 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
     face_flux<4>(b.rs, UL, UR, 0, F, sm);
 }
 
-// wave speed of the two (equal) y faces of cell i
+// wave speed of the two (equal) y faces of cell i: the speed face_flux(u, u, y) returns for the cell's face state u,
+// without its flux. Both states are u and the normal velocity is +0 (rho v = +0), so the Einfeldt bounds are
+// S_L = 0 - c = -c and S_R = c, with c the sound speed of c2p(u), and max(|S_L|, |S_R|) = c; the low-Mach variant
+// takes c before its velocity blend, which leaves equal states unchanged.
 template <int RECON>
 __device__ __forceinline__ double by_speed(const BP& b, const BInst& in, const double* Us, int i) {
-    double u[4], F[4], sm;
+    double u[4], w[4], inv;
     bload_cons(b, in, Us, i, u);
     if constexpr (RECON != 0) roundtrip<4>(b.rs.eos, u);
-    face_flux<4>(b.rs, u, u, 1, F, sm);
-    return sm;
+    c2p<4>(b.rs.eos, u, w, &inv);
+    return sound_speed_inv(b.rs.eos, w[3], inv);
 }
 
 // maximum over the workgroup of non-negative, non-NaN doubles; every thread gets it
@@ -279,6 +283,218 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
 }
 
 
+// ------------------------------------------------------------------------------------------------------------
+// 2-D instances (fvg_batch_create_2d): instance i is the reference's run on an nx_i x ny_i mesh with the four sides,
+// Lx and Ly of the shared configuration; the same persistent layout (one workgroup per instance, its whole run),
+// with the 2-D path's arithmetic (NV = 4): primitives, x and y faces, wave-speed maximum, dt, balance and stage
+// combination per stage. Halos as the padded planes of the 2-D path hold them (push_halo): periodic wrap, ghost slab
+// (boundary_order = 2: the ghost state of the mirrored cell), or the edge cell.
+// ------------------------------------------------------------------------------------------------------------
+struct BInst2 {
+    long long c0, fx0, fy0;   // first cell, first x face and first y face of the instance in the planes
+    int nx, ny;
+    int halo[4];              // per side: 0 none, 1 periodic wrap, 2 ghost slab
+    double Ax, Ay, inv_vol, Ldt;
+};
+
+struct BP2 {
+    RiemannParams rs;
+    BcParams bp;
+    int recon, lim, scheme, max_steps;
+    double weno_eps, cfl, t_end, dt_max;
+    long long nc, nf;   // plane strides: cells and faces (x and y) of the whole ensemble
+    double* U;          // 4 planes (rho, rho u, rho v, E)
+    double* Ut;
+    double* W;          // 4 planes (rho, u, v, p)
+    double* F;          // 4 planes: the x faces then the y faces of each instance
+    const BInst2* inst;
+    const int* order;
+    BOut* out;
+};
+
+__device__ __forceinline__ void b2load(const double* P, long long stride, long long c, double* v) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = P[q * stride + c];
+}
+
+// the padded primitive state at coordinate t along axis d of the line through (i, j) (stencil_prim of the 2-D path)
+__device__ __forceinline__ void b2stencil(const BP2& b, const BInst2& in, const double* Us, int d, int i, int j, int t, double* w) {
+    const int n = d == 0 ? in.nx : in.ny;
+    auto cell = [&](int tt) { return in.c0 + (long long)(d == 1 ? tt : j) * in.nx + (d == 0 ? tt : i); };
+    if (t >= 0 && t < n) {
+        b2load(b.W, b.nc, cell(t), w);
+        return;
+    }
+    const int hi = t >= n;
+    const int h = in.halo[2 * d + hi];
+    if (h == 1) {
+        b2load(b.W, b.nc, cell(hi ? t - n : t + n), w);
+    } else if (h == 2) {
+        const int l = hi ? t - n : -1 - t;
+        double ui[4], ug[4];
+        b2load(Us, b.nc, cell(hi ? n - 1 - l : l), ui);
+        ghost_state<4>(b.rs.eos, b.bp, 2 * d + hi, ui, ug);
+        c2p<4>(b.rs.eos, ug, w, (double*)nullptr);
+    } else {
+        b2load(b.W, b.nc, cell(t < 0 ? 0 : n - 1), w);
+    }
+}
+
+// the first-order state beyond side 2d + hi of the boundary cell of the line through (i, j) (outside_first_order)
+__device__ __forceinline__ void b2outside(const BP2& b, const BInst2& in, const double* Us, int d, int hi, int i, int j, double* ug) {
+    const int n = d == 0 ? in.nx : in.ny, side = 2 * d + hi;
+    auto cell = [&](int tt) { return in.c0 + (long long)(d == 1 ? tt : j) * in.nx + (d == 0 ? tt : i); };
+    if (bc_of(b.bp, side) == 7) {
+        b2load(Us, b.nc, cell(hi ? 0 : n - 1), ug);
+        return;
+    }
+    double ui[4];
+    b2load(Us, b.nc, cell(hi ? n - 1 : 0), ui);
+    ghost_state<4>(b.rs.eos, b.bp, side, ui, ug);
+}
+
+// face t along axis d of the line through (i, j): the states of face_states (MODE_PLAIN / MODE_FIRST_ORDER) and the
+// Riemann flux
+template <int RECON>
+__device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const double* Us, int d, int i, int j, int t, double* F, double& sm) {
+    const int n = d == 0 ? in.nx : in.ny;
+    const bool lo = t == 0, hi = t == n;
+    auto cell = [&](int tt) { return in.c0 + (long long)(d == 1 ? tt : j) * in.nx + (d == 0 ? tt : i); };
+    double UL[4], UR[4];
+    const bool upgrade = RECON != 0 && ((t > 0 && t < n) || (lo && in.halo[2 * d]) || (hi && in.halo[2 * d + 1]));
+    if (upgrade) {
+        double wl[4], wr[4];
+        const double we = b.weno_eps;
+        if constexpr (RECON == 3) {
+            double c[6][4];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) b2stencil(b, in, Us, d, i, j, t - 3 + s, c[s]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+            }
+        } else {
+            double c[4][4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) b2stencil(b, in, Us, d, i, j, t - 2 + s, c[s]);
+            if constexpr (RECON == 2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
+                    wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    wl[q] = c[1][q] + 0.5 * limiter(b.lim, c[1][q] - c[0][q], c[2][q] - c[1][q]);
+                    wr[q] = c[2][q] - 0.5 * limiter(b.lim, c[3][q] - c[2][q], c[2][q] - c[1][q]);
+                }
+            }
+        }
+        p2c<4>(b.rs.eos, wl, UL);
+        p2c<4>(b.rs.eos, wr, UR);
+    } else {
+        if (!lo) b2load(Us, b.nc, cell(t - 1), UL);
+        else b2outside(b, in, Us, d, 0, i, j, UL);
+        if (!hi) b2load(Us, b.nc, cell(t), UR);
+        else b2outside(b, in, Us, d, 1, i, j, UR);
+        if constexpr (RECON != 0) {   // a boundary face without a halo: the primitive round trip
+            roundtrip<4>(b.rs.eos, UL);
+            roundtrip<4>(b.rs.eos, UR);
+        }
+    }
+    face_flux<4>(b.rs, UL, UR, d, F, sm);
+}
+
+template <int BS, int RECON>
+__global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
+    __shared__ double red[BS / 64];
+    const int id = b.order[blockIdx.x];
+    const BInst2 in = b.inst[id];
+    const int nx = in.nx, ny = in.ny, n = nx * ny;
+    const int nfx = (nx + 1) * ny, nfy = nx * (ny + 1);
+    const int nst = b.scheme + 1;
+    double t = 0.0, dt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s_step = 0.0;
+    int steps = 0, diverged = 0;
+    while (t < b.t_end && steps < b.max_steps) {
+        for (int st = 1; st <= nst; ++st) {
+            const double* Us = st == 1 ? b.U : b.Ut;
+            for (int c = threadIdx.x; c < n; c += BS) {   // primitives of the stage input (k_cells)
+                double u[4], w[4];
+                b2load(Us, b.nc, in.c0 + c, u);
+                c2p<4>(b.rs.eos, u, w, (double*)nullptr);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b.W[q * b.nc + in.c0 + c] = w[q];
+            }
+            __syncthreads();
+            double local = 0.0;
+            for (int f = threadIdx.x; f < nfx + nfy; f += BS) {   // x faces, then y faces (k_faces)
+                const int d = f < nfx ? 0 : 1;
+                const int g = d == 0 ? f : f - nfx;
+                const int w = d == 0 ? nx + 1 : nx;
+                const int j = g / w, i = g - j * w;   // x faces: face i of row j; y faces: face j of column i
+                double F[4], sm;
+                b2face<RECON>(b, in, Us, d, d == 0 ? 0 : i, d == 0 ? j : 0, d == 0 ? i : j, F, sm);
+                if (sm > local) local = sm;
+                const long long x = (d == 0 ? in.fx0 : in.fy0) + g;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b.F[q * b.nf + x] = F[q];
+            }
+            const double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            if (st == 1) {   // k_control
+                s1 = S;
+                dt = smin(b.cfl * in.Ldt / (S + kDtEps), b.t_end - t);
+                if (b.dt_max > 0.0) dt = smin(dt, b.dt_max);
+            } else if (st == 2) {
+                s2 = S;
+            } else {
+                s3 = S;
+            }
+            for (int c = threadIdx.x; c < n; c += BS) {   // balance and stage combination (k_update)
+                const int j = c / nx, i = c - j * nx;
+                const long long fx = in.fx0 + (long long)j * (nx + 1) + i, fy = in.fy0 + (long long)j * nx + i;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double* Fq = b.F + q * b.nf;
+                    const double fl = Fq[fx], fr = Fq[fx + 1], fb = Fq[fy], ft = Fq[fy + nx];
+                    const double R = ((0.0 - (fr * in.Ax - fl * in.Ax)) - (ft * in.Ay - fb * in.Ay)) * in.inv_vol;
+                    const long long y = q * b.nc + in.c0 + c;
+                    if (b.scheme == 0) {
+                        b.U[y] = b.U[y] + dt * R;
+                    } else if (st == 1) {
+                        b.Ut[y] = b.U[y] + dt * R;
+                    } else if (b.scheme == 1) {
+                        b.U[y] = 0.5 * b.U[y] + 0.5 * (b.Ut[y] + dt * R);
+                    } else if (st == 2) {
+                        b.Ut[y] = 0.75 * b.U[y] + 0.25 * (b.Ut[y] + dt * R);
+                    } else {
+                        b.U[y] = (1.0 / 3.0) * b.U[y] + (2.0 / 3.0) * (b.Ut[y] + dt * R);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        s_step = b.scheme == 0 ? s1 : (b.scheme == 1 ? smax(s1, s2) : smax(s1, smax(s2, s3)));
+        steps += 1;
+        if (!(isfinite(s_step) && isfinite(dt) && t + dt > t)) {   // check_progress (SimulationInternal.hpp:259-267)
+            diverged = 1;
+            break;
+        }
+        t += dt;
+    }
+    if (threadIdx.x == 0) {
+        BOut o;
+        o.t = t;
+        o.s_last = s_step;
+        o.dt_last = dt;
+        o.steps = steps;
+        o.diverged = diverged;
+        b.out[id] = o;
+    }
+}
+
+
 }   // namespace fvg
 
 using namespace fvg;
@@ -294,10 +510,15 @@ struct fvg_batch {
     hipStream_t stream = nullptr;
     int ninst = 0;
     long long nc = 0, nf = 0;
-    std::vector<int> n;
+    int dim = 1;                  // 1: N x 1 instances (3 planes), 2: nx x ny instances (4 planes)
+    int nplanes = 3;
+    std::vector<int> n;           // cells per instance
     std::vector<BInst> inst;
+    std::vector<BInst2> inst2;
     std::vector<int> order;
     BInst* d_inst = nullptr;
+    BInst2* d_inst2 = nullptr;
+    BP2 base2{};
     int* d_order = nullptr;
     BOut* d_out = nullptr;
     double *U = nullptr, *Ut = nullptr, *W = nullptr, *F = nullptr, *Usave = nullptr;
@@ -325,6 +546,7 @@ int bset(fvg_batch* b, int code, const std::string& m) {
 void free_batch(fvg_batch* b) {
     if (!b) return;
     hipFree(b->d_inst);
+    hipFree(b->d_inst2);
     hipFree(b->d_order);
     hipFree(b->d_out);
     hipFree(b->U);
@@ -339,6 +561,19 @@ void free_batch(fvg_batch* b) {
 template <int BS>
 void launch_bs(fvg_batch* b, const BP& p) {
     const dim3 g(b->ninst), t(BS);
+    if (b->dim == 2) {   // the run's time, dt_max and step limit go with the 2-D parameters
+        BP2 q = b->base2;
+        q.t_end = p.t_end;
+        q.dt_max = p.dt_max;
+        q.max_steps = p.max_steps;
+        switch (q.recon) {
+        case 0: hipLaunchKernelGGL((k_ensemble2<BS, 0>), g, t, 0, b->stream, q); break;
+        case 1: hipLaunchKernelGGL((k_ensemble2<BS, 1>), g, t, 0, b->stream, q); break;
+        case 2: hipLaunchKernelGGL((k_ensemble2<BS, 2>), g, t, 0, b->stream, q); break;
+        default: hipLaunchKernelGGL((k_ensemble2<BS, 3>), g, t, 0, b->stream, q); break;
+        }
+        return;
+    }
     switch (p.recon) {
     case 0: hipLaunchKernelGGL((k_ensemble<BS, 0>), g, t, 0, b->stream, p); break;
     case 1: hipLaunchKernelGGL((k_ensemble<BS, 1>), g, t, 0, b->stream, p); break;
@@ -391,6 +626,10 @@ std::string tune_key(const fvg_batch* b) {
     unsigned long long h = 1469598103934665603ull;   // FNV-1a over the instance sizes
     for (int v : b->n) {
         h ^= (unsigned long long)v;
+        h *= 1099511628211ull;
+    }
+    for (const BInst2& in : b->inst2) {   // 2-D instances: their shapes
+        h ^= (unsigned long long)in.nx * 1000003ull + (unsigned long long)in.ny;
         h *= 1099511628211ull;
     }
     std::ostringstream s;
@@ -505,31 +744,136 @@ int fvg_batch_destroy(fvg_batch* b) {
     return FVG_OK;
 }
 
+int fvg_batch_create_2d(const fvg_config* cfg_in, int ninst, const int* nx, const int* ny, int device, fvg_batch** out) {
+    if (!cfg_in || !nx || !ny || !out) return bset(nullptr, FVG_EINVAL, "null argument");
+    *out = nullptr;
+    if (ninst <= 0) return bset(nullptr, FVG_EINVAL, "ninst must be positive");
+    fvg_config cfg = *cfg_in;
+    cfg.nx = nx[0];
+    cfg.ny = ny[0];
+    cfg.nz = 1;
+    char msg[256] = {0};
+    if (fvg_config_finalize(&cfg, msg, sizeof msg) != FVG_OK) return bset(nullptr, FVG_EINVAL, msg);
+    for (int s = 0; s < 4; ++s) {   // the side types whose ghost states need no other cell of the side
+        const int bt = cfg.bc[s];
+        if (bt == FVG_BC_MASSFLOW_INLET || bt == FVG_BC_NONREFLECTING)
+            return bset(nullptr, FVG_EUNSUPPORTED, "mass-flow inlet and non-reflecting sides are not in the ensemble");
+    }
+    if (cfg.use_viscous) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble runs the Euler equations (use_viscous = false)");
+    if (cfg.time_scheme == FVG_TS_HANCOCK) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble runs explicit Euler or SSP Runge-Kutta schemes");
+    if (cfg.flux_precision != FVG_FLUX_FP64) return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble computes in FP64");
+    if (cfg.flux_quadrature != FVG_QUAD_MIDPOINT || cfg.indicator != FVG_INDICATOR_NONE)
+        return bset(nullptr, FVG_EUNSUPPORTED, "the ensemble takes the midpoint quadrature and no indicator");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bset(nullptr, FVG_EHIP, "no HIP device");
+    if (device < 0 || device >= ndev) return bset(nullptr, FVG_EINVAL, "device index out of range");
+
+    fvg_batch* b = new fvg_batch();
+    b->cfg = cfg;
+    b->device = device;
+    b->ninst = ninst;
+    b->dim = 2;
+    b->nplanes = 4;
+    b->n.resize(ninst);
+    b->inst2.resize(ninst);
+    long long c0 = 0, f0 = 0;
+    for (int i = 0; i < ninst; ++i) {
+        if (nx[i] < 1 || ny[i] < 1 || (long long)nx[i] * ny[i] > (1 << 24)) {
+            free_batch(b);
+            return bset(nullptr, FVG_EINVAL, "instance " + std::to_string(i) + ": mesh size out of range");
+        }
+        fvg_config ci = cfg;
+        ci.nx = nx[i];
+        ci.ny = ny[i];
+        KP kp{};
+        fill_kernel_params(ci, kp);
+        BInst2& in = b->inst2[i];
+        in.c0 = c0;
+        in.fx0 = f0;
+        in.fy0 = f0 + (long long)(nx[i] + 1) * ny[i];
+        in.nx = nx[i];
+        in.ny = ny[i];
+        for (int s = 0; s < 4; ++s) in.halo[s] = kp.halo[s];
+        in.Ax = kp.Ax;
+        in.Ay = kp.Ay;
+        in.inv_vol = kp.inv_vol;
+        in.Ldt = kp.Ldt;
+        b->n[i] = nx[i] * ny[i];
+        c0 += (long long)nx[i] * ny[i];
+        f0 += (long long)(nx[i] + 1) * ny[i] + (long long)nx[i] * (ny[i] + 1);
+    }
+    b->nc = c0;
+    b->nf = f0;
+    b->order.resize(ninst);
+    for (int i = 0; i < ninst; ++i) b->order[i] = i;
+    auto fail = [&](int code, const std::string& m) {
+        free_batch(b);
+        return bset(nullptr, code, m);
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(FVG_EHIP, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(FVG_EHIP, "hipStreamCreate failed");
+    const size_t cb = sizeof(double) * 4 * (size_t)b->nc, fb = sizeof(double) * 4 * (size_t)b->nf;
+    if (hipMalloc(&b->U, cb) != hipSuccess || hipMalloc(&b->Ut, cb) != hipSuccess || hipMalloc(&b->W, cb) != hipSuccess ||
+        hipMalloc(&b->Usave, cb) != hipSuccess || hipMalloc(&b->F, fb) != hipSuccess ||
+        hipMalloc(&b->d_inst2, sizeof(BInst2) * ninst) != hipSuccess || hipMalloc(&b->d_order, sizeof(int) * ninst) != hipSuccess ||
+        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess)
+        return fail(FVG_ENOMEM, "hipMalloc of the ensemble buffers");
+    if (hipMemcpy(b->d_inst2, b->inst2.data(), sizeof(BInst2) * ninst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_order, b->order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(FVG_EHIP, "hipMemcpy of the instance table");
+    KP kp{};
+    fill_kernel_params(cfg, kp);
+    BP2& p = b->base2;
+    p.rs = kp.rs;
+    p.bp = kp.bp;
+    p.recon = cfg.reconstruction;
+    p.lim = cfg.limiter;
+    p.scheme = cfg.time_scheme == FVG_TS_EULER ? 0 : (cfg.time_scheme == FVG_TS_SSPRK2 ? 1 : 2);
+    p.weno_eps = cfg.weno_epsilon;
+    p.cfl = cfg.cfl;
+    p.nc = b->nc;
+    p.nf = b->nf;
+    p.U = b->U;
+    p.Ut = b->Ut;
+    p.W = b->W;
+    p.F = b->F;
+    p.inst = b->d_inst2;
+    p.order = b->d_order;
+    p.out = b->d_out;
+    *out = b;
+    return FVG_OK;
+}
+
 int fvg_batch_upload(fvg_batch* b, const double* U, long long ncells, int ncomp) {
     if (!b || !U) return bset(b, FVG_EINVAL, "null argument");
     if (ncells != b->nc) return bset(b, FVG_EINVAL, "cell count differs from the ensemble's total");
-    if (ncomp != 3 && ncomp != 4 && ncomp != 5) return bset(b, FVG_EINVAL, "ncomp must be 3, 4 or 5");
-    // (rho, rho u, E) planes; the estimated cost of each instance orders the workgroups (longest first)
-    const int e = ncomp == 3 ? 2 : 3;
-    std::vector<double> h((size_t)3 * b->nc);
+    if (b->dim == 2 ? (ncomp != 4 && ncomp != 5) : (ncomp != 3 && ncomp != 4 && ncomp != 5))
+        return bset(b, FVG_EINVAL, b->dim == 2 ? "ncomp must be 4 or 5" : "ncomp must be 3, 4 or 5");
+    // the planes ((rho, rho u, E), or (rho, rho u, rho v, E) for 2-D instances); the estimated cost of each instance
+    // (cells x the cells along its longer axis x the largest initial wave speed) orders the workgroups, longest first
+    const int np = b->nplanes, e = ncomp == 3 ? 2 : 3;
+    std::vector<double> h((size_t)np * b->nc);
     const double g = b->cfg.gamma, pinf = b->cfg.eos_type == FVG_EOS_STIFFENED ? b->cfg.p_inf : 0.0;
     std::vector<double> cost(b->ninst);
     for (int i = 0; i < b->ninst; ++i) {
-        const BInst& in = b->inst[i];
+        const long long c0 = b->dim == 2 ? b->inst2[i].c0 : b->inst[i].c0;
+        const int nc_i = b->n[i];
+        const int lmax = b->dim == 2 ? std::max(b->inst2[i].nx, b->inst2[i].ny) : nc_i;
         double smax0 = 0.0;
-        for (int k = 0; k < in.n; ++k) {
-            const long long c = in.c0 + k;
+        for (int k = 0; k < nc_i; ++k) {
+            const long long c = c0 + k;
             const double* s = U + (size_t)ncomp * c;
             h[c] = s[0];
             h[b->nc + c] = s[1];
-            h[2 * b->nc + c] = s[e];
-            const double rho = std::max(s[0], 1e-10), u = s[1] / rho;
-            const double p = (g - 1.0) * (s[e] - 0.5 * rho * u * u) - (pinf != 0.0 ? g * pinf : 0.0);
+            if (np == 4) h[2 * b->nc + c] = s[2];
+            h[(np - 1) * b->nc + c] = s[e];
+            const double rho = std::max(s[0], 1e-10), u = s[1] / rho, v = np == 4 ? s[2] / rho : 0.0;
+            const double p = (g - 1.0) * (s[e] - 0.5 * rho * (u * u + v * v)) - (pinf != 0.0 ? g * pinf : 0.0);
             const double c2 = g * (std::max(p, 1e-10) + pinf) / rho;
-            const double sp = std::fabs(u) + std::sqrt(std::max(c2, 0.0));
+            const double sp = std::sqrt(u * u + v * v) + std::sqrt(std::max(c2, 0.0));
             if (std::isfinite(sp) && sp > smax0) smax0 = sp;
         }
-        cost[i] = (double)in.n * in.n * (smax0 + 1e-3);
+        cost[i] = (double)nc_i * lmax * (smax0 + 1e-3);
     }
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
     BCHK(b, hipSetDevice(b->device));
@@ -542,15 +886,20 @@ int fvg_batch_upload(fvg_batch* b, const double* U, long long ncells, int ncomp)
 int fvg_batch_download(fvg_batch* b, double* U, long long ncells, int ncomp) {
     if (!b || !U) return bset(b, FVG_EINVAL, "null argument");
     if (ncells != b->nc) return bset(b, FVG_EINVAL, "cell count differs from the ensemble's total");
-    if (ncomp != 3 && ncomp != 4 && ncomp != 5) return bset(b, FVG_EINVAL, "ncomp must be 3, 4 or 5");
-    std::vector<double> h((size_t)3 * b->nc);
+    if (b->dim == 2 ? (ncomp != 4 && ncomp != 5) : (ncomp != 3 && ncomp != 4 && ncomp != 5))
+        return bset(b, FVG_EINVAL, b->dim == 2 ? "ncomp must be 4 or 5" : "ncomp must be 3, 4 or 5");
+    std::vector<double> h((size_t)b->nplanes * b->nc);
     BCHK(b, hipSetDevice(b->device));
     BCHK(b, hipMemcpy(h.data(), b->U, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (long long c = 0; c < b->nc; ++c) {
         double* s = U + (size_t)ncomp * c;
         s[0] = h[c];
         s[1] = h[b->nc + c];
-        if (ncomp == 3) {
+        if (b->nplanes == 4) {
+            s[2] = h[2 * b->nc + c];
+            s[3] = h[3 * b->nc + c];
+            if (ncomp == 5) s[4] = 0.0;
+        } else if (ncomp == 3) {
             s[2] = h[2 * b->nc + c];
         } else {
             s[2] = 0.0;
@@ -622,7 +971,7 @@ int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cach
         }
     }
     BCHK(b, hipSetDevice(b->device));
-    const size_t cb = sizeof(double) * 3 * (size_t)b->nc;
+    const size_t cb = sizeof(double) * b->nplanes * (size_t)b->nc;
     BCHK(b, hipMemcpyAsync(b->Usave, b->U, cb, hipMemcpyDeviceToDevice, b->stream));
     int best = b->block;
     double best_ms = 1e300;

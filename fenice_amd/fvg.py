@@ -32,7 +32,7 @@ EXPORTS = [
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
     "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
     "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read", "fvg_monitor",
-    "fvg_conserved_totals_noalloc",
+    "fvg_conserved_totals_noalloc", "fvg_batch_create_2d",
 ]
 
 MLP_NPARAM = 1601
@@ -108,6 +108,7 @@ def lib():
         L.fvg_set_fused.argtypes = [C.c_void_p, C.c_int]
         L.fvg_riemann_batch.argtypes = [P, C.c_int, _D, _D, _I, C.c_size_t, _D, _D]
         L.fvg_batch_create.argtypes = [P, C.c_int, _I, C.c_int, C.POINTER(C.c_void_p)]
+        L.fvg_batch_create_2d.argtypes = [P, C.c_int, _I, _I, C.c_int, C.POINTER(C.c_void_p)]
         L.fvg_batch_destroy.argtypes = [C.c_void_p]
         L.fvg_batch_last_error.argtypes = [C.c_void_p]
         L.fvg_batch_last_error.restype = C.c_char_p
@@ -341,11 +342,21 @@ class Ensemble:
     kernel launch (fvg_batch_*). `cfg` gives the scheme, x boundaries, Lx and Ly shared by every instance; its
     ny must be 1 with periodic y sides."""
 
-    def __init__(self, cfg: Config, ncells, device: int = 0):
-        self.n = np.ascontiguousarray(ncells, dtype=np.int32)
+    def __init__(self, cfg: Config, ncells, device: int = 0, ny=None):
+        """ncells: cells per instance (N x 1 meshes); with ny, ncells are the nx of 2-D instances nx[i] x ny[i]."""
+        if ny is None:
+            self.n = np.ascontiguousarray(ncells, dtype=np.int32)
+        else:
+            self.nx = np.ascontiguousarray(ncells, dtype=np.int32)
+            self.ny = np.ascontiguousarray(ny, dtype=np.int32)
+            self.n = (self.nx * self.ny).astype(np.int32)
         self.total = int(self.n.sum())
         h = C.c_void_p()
-        rc = lib().fvg_batch_create(C.byref(cfg), int(self.n.size), self.n.ctypes.data_as(_I), device, C.byref(h))
+        if ny is None:
+            rc = lib().fvg_batch_create(C.byref(cfg), int(self.n.size), self.n.ctypes.data_as(_I), device, C.byref(h))
+        else:
+            rc = lib().fvg_batch_create_2d(C.byref(cfg), int(self.n.size), self.nx.ctypes.data_as(_I),
+                                           self.ny.ctypes.data_as(_I), device, C.byref(h))
         if rc != OK:
             raise _ERR.get(rc, FvgError)(lib().fvg_batch_last_error(None).decode(errors="replace"))
         self._h = h

@@ -156,6 +156,41 @@ def vortex_state(n: int, L: float = 10.0, gamma: float = 1.4) -> np.ndarray:
     return U
 
 
+def vortex_rect(nx: int, ny: int, L: float = 10.0, gamma: float = 1.4) -> np.ndarray:
+    """(nx*ny, 4) conservative state of the isentropic vortex (beta 5, free stream (1, 1), centre (L/2, L/2)) on an
+    nx x ny mesh of [0, L]^2 (cpp/src/TestCases.cpp:284-295; vortex_state is the square case)."""
+    hx, hy = L / nx, L / ny
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    dx = (i.ravel() + 0.5) * hx - 0.5 * L
+    dy = (j.ravel() + 0.5) * hy - 0.5 * L
+    r2 = dx * dx + dy * dy
+    g = 5.0 / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (gamma - 1.0) * 25.0 / (8.0 * gamma * np.pi * np.pi) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    u, v, p = 1.0 - g * dy, 1.0 + g * dx, rho * T
+    U = np.zeros((nx * ny, 4))
+    U[:, 0], U[:, 1], U[:, 2] = rho, rho * u, rho * v
+    U[:, 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def vortex_ensemble(ninst: int = 256, seed: int = 256, nmin: int = 32, nmax: int = 256):
+    """Isentropic vortices on square periodic meshes of uneven resolution, n uniform in [nmin, nmax]:
+    (n int32[ninst], U float64[sum n^2, 4])."""
+    r = unit_doubles(seed, ninst)
+    n = (nmin + np.floor(r * (nmax - nmin + 1))).astype(np.int32)
+    return n, np.concatenate([vortex_rect(int(k), int(k)) for k in n])
+
+
+def vortex_ensemble_keys(**over) -> dict:
+    """MUSCL (Van Leer) + HLLC + SSP-RK2, CFL 0.5, periodic [0, 10]^2 (the mesh size is each instance's)."""
+    k = dict(nx="8", ny="8", nz="1", Lx="10.0", Ly="10.0", gamma="1.4", use_viscous="false", bc_xmin="periodic",
+             bc_xmax="periodic", bc_ymin="periodic", bc_ymax="periodic", riemann_solver="HLLC", reconstruction="muscl",
+             muscl_limiter="Van Leer", time_scheme="SSP Runge-Kutta 2", cfl="0.5", boundary_order="1")
+    k.update(over)
+    return k
+
+
 def vortex_keys(n: int, cfl: str = "0.8", time_scheme: str = "MUSCL-Hancock") -> dict:
     """MUSCL (Van Leer) + HLLC, periodic [0, 10]^2."""
     return dict(nx=str(n), ny=str(n), nz="1", Lx="10.0", Ly="10.0", gamma="1.4", use_viscous="false", bc_xmin="periodic",

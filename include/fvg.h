@@ -286,6 +286,11 @@ int fvg_mlp_logits(int device, const float* params, float threshold, const float
 typedef struct fvg_batch fvg_batch;
 
 int fvg_batch_create(const fvg_config* cfg, int ninst, const int* ncells, int device, fvg_batch** out);
+/* An ensemble of 2-D runs: instance i on an nx[i] x ny[i] mesh with the scheme, the four sides (any but mass-flow
+ * inlets and non-reflecting outlets), Lx and Ly of cfg, e.g. isentropic vortices of uneven resolution. The same
+ * persistent kernel layout (one workgroup per instance); the instances keep (rho, rho u, rho v, E), and upload and
+ * download take ncomp 4 or 5 (rho w written as +0). The other fvg_batch_* entries apply unchanged. */
+int fvg_batch_create_2d(const fvg_config* cfg, int ninst, const int* nx, const int* ny, int device, fvg_batch** out);
 int fvg_batch_destroy(fvg_batch* b);
 /* Message of the last failure of b (of the last failed fvg_batch_create when b is NULL). */
 const char* fvg_batch_last_error(const fvg_batch* b);

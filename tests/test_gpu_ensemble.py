@@ -1,12 +1,13 @@
-"""The 1-D ensemble kernel (fvg_batch_*) against the oracle's N x 1 runs (the reference's mapping of a 1-D
-problem): every instance's state, time and step count bit for bit, over schemes, boundaries and sizes down to
-single cells, independent of the workgroup size, with per-instance divergence."""
+"""The ensemble kernels (fvg_batch_*) against the oracle: 1-D instances as the reference's N x 1 runs (its mapping
+of a 1-D problem), 2-D instances (isentropic vortices on meshes of uneven shape) as its 2-D runs; every instance's
+state, time and step count bit for bit, over schemes, boundaries and sizes down to single cells, independent of the
+workgroup size, with per-instance divergence."""
 import numpy as np
 import pytest
 
 from cases import oracle_instance
 from fenice_amd import fvg
-from fenice_amd.workloads import ensemble_keys, riemann_ensemble, riemann_states
+from fenice_amd.workloads import ensemble_keys, riemann_ensemble, riemann_states, vortex_ensemble_keys, vortex_rect
 
 pytestmark = pytest.mark.gpu
 
@@ -140,3 +141,79 @@ def test_rejects_unsupported_configurations():
             e.run(0.1, 0)
         with pytest.raises(fvg.InvalidArgument):
             e.upload(np.ones((16, 3)))
+
+
+# ---- 2-D instances (fvg_batch_create_2d) ---------------------------------------------------------------------------
+def oracle_instance_2d(oracle, keys, nx, ny, U4, t_end, max_steps):
+    cfg = fvg.config_from_keys(dict(keys, nx=str(nx), ny=str(ny)))
+    U = np.zeros((nx * ny, 5))
+    U[:, :4] = U4
+    t, steps, diverged = 0.0, 0, 0
+    while t < t_end and steps < max_steps:
+        dt, _, _, ok = oracle.step(cfg, U, t, t_end, 0.0)
+        steps += 1
+        if not ok:
+            diverged = 1
+            break
+        t += dt
+    return U[:, :4].copy(), t, steps, diverged
+
+
+SIZES_2D = [(24, 24), (13, 11), (40, 32), (8, 8), (5, 3), (2, 2), (1, 7), (33, 17)]
+SCHEMES_2D = [
+    ("vortex-muscl-rk2-cfl0.8", dict(cfl="0.8")),   # some instances diverge, as in the oracle
+    ("weno5-rk3", dict(reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5")),
+    ("weno3-walls-b2", dict(reconstruction="weno3", time_scheme="SSP Runge-Kutta 3", cfl="0.5", bc_xmin="wall", bc_xmax="no_slip",
+                            bc_ymin="transmissive", bc_ymax="outlet", p_out="0.9", boundary_order="2")),
+    ("inlet-moving-wall", dict(muscl_limiter="MC", bc_xmin="vel_inlet", bc_xmax="transmissive", bc_ymin="wall",
+                               bc_ymax="moving_wall", u_in="0.5", v_in="0.1", boundary_order="2", cfl="0.5")),
+    ("first-order-euler", dict(reconstruction="none", time_scheme="Explicit Euler", cfl="0.4")),
+    ("lowmach-blend", dict(riemann_solver="HLLC (Low-Mach Thornber)", shock_blend="0.2", muscl_limiter="Koren", cfl="0.5")),
+]
+
+
+@pytest.mark.parametrize("name,over", SCHEMES_2D, ids=[s[0] for s in SCHEMES_2D])
+def test_2d_instances_match_oracle(oracle, name, over):
+    # isentropic vortices on meshes of uneven shape, down to single rows and columns, each against the oracle's 2-D
+    # run of its mesh: state, time and step count bit for bit
+    say("ensemble 2-D", name)
+    keys = vortex_ensemble_keys(**over)
+    nx, ny = np.array([s[0] for s in SIZES_2D]), np.array([s[1] for s in SIZES_2D])
+    U = np.concatenate([vortex_rect(a, b) for a, b in SIZES_2D])
+    t_end, max_steps = 1.0, 40
+    with fvg.Ensemble(fvg.config_from_keys(keys), nx, ny=ny) as e:
+        e.upload(U)
+        r = e.run(t_end, max_steps)
+        Ud = e.download(4)
+        off = e.offsets
+    for i, (a, b) in enumerate(SIZES_2D):
+        Uo, t, steps, div = oracle_instance_2d(oracle, keys, a, b, U[off[i]:off[i + 1]], t_end, max_steps)
+        assert steps == r.steps[i] and div == r.diverged[i], (i, steps, r.steps[i], div, r.diverged[i])
+        assert t == r.t[i], (i, t, r.t[i])
+        assert same(Ud[off[i]:off[i + 1]], Uo), (i, a, b)
+
+
+def test_2d_block_sizes_and_tuner(oracle):
+    # the workgroup size leaves the bits unchanged; the tuner restores the uploaded state
+    keys = vortex_ensemble_keys()
+    sizes = [(40, 40), (17, 29), (64, 48)]
+    nx, ny = np.array([s[0] for s in sizes]), np.array([s[1] for s in sizes])
+    U = np.concatenate([vortex_rect(a, b) for a, b in sizes])
+    outs = []
+    for block in (64, 256, 1024):
+        with fvg.Ensemble(fvg.config_from_keys(keys), nx, ny=ny) as e:
+            e.block = block
+            e.upload(U)
+            if block == 1024:
+                e.tune(0.05, 100)
+                assert same(e.download(4), U)
+            e.run(0.5, 30)
+            outs.append(e.download(4))
+    assert same(outs[0], outs[1]) and same(outs[0], outs[2])
+
+
+def test_2d_rejects_unsupported_sides():
+    with pytest.raises(fvg.Unsupported):
+        fvg.Ensemble(fvg.config_from_keys(vortex_ensemble_keys(bc_xmin="massflow_inlet", bc_xmax="transmissive")), [8], ny=[8])
+    with pytest.raises(fvg.Unsupported):
+        fvg.Ensemble(fvg.config_from_keys(vortex_ensemble_keys(use_viscous="true")), [8], ny=[8])
