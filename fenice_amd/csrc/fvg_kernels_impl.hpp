@@ -952,14 +952,20 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
         float x[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) x[q] = 0.0f;
-        if (upgrade) {
+        if (upgrade) {   // the stencil for the features only: it is loaded again below, so that it is not held in
+                         // registers across the network (with FP64 stencils that spilled)
+            T cf[6][NV];
 #pragma unroll
-            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
-            face_features<NV, T>(c, float(p.weno_eps), x);
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, cf[s]);
+            face_features<NV, T>(cf, float(p.weno_eps), x);
         }
         const float z = mlp_logit_wave(p.mlp, x);   // every lane of the wavefront takes part
         troubled = z > p.mlp->thr;
-        if (upgrade) kept_of<NV, D>(p)[f] = troubled ? 1 : 0;
+        if (upgrade) {
+            kept_of<NV, D>(p)[f] = troubled ? 1 : 0;
+#pragma unroll
+            for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
+        }
     }
     if (!active) return;
     const bool lo = t == 0, hi = t == n;
