@@ -882,7 +882,8 @@ int fvg_create(const fvg_config* cfg_in, int device, fvg_ctx** out) {
                 rc = set_err(c, FVG_EHIP, "hipMemcpy of the indicator state");
         }
     }
-    if (!rc && hipMalloc(&c->st, sizeof(DevState)) != hipSuccess) rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
+    if (!rc && (hipMalloc(&c->st, sizeof(DevState)) != hipSuccess || hipMemset(c->st, 0, sizeof(DevState)) != hipSuccess))
+        rc = set_err(c, FVG_ENOMEM, "hipMalloc of the device state");
     const size_t spd_bytes = sizeof(unsigned long long) * 2 * kSpeedSlots * kSlotStride;
     if (!rc && (hipMalloc(&c->spd, spd_bytes) != hipSuccess || hipMemset(c->spd, 0, spd_bytes) != hipSuccess))
         rc = set_err(c, FVG_ENOMEM, "hipMalloc of the wave-speed slots");

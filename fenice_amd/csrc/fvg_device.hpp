@@ -33,6 +33,28 @@ __device__ __forceinline__ double dv(double a, double b) { return a / b; }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sq(double a) { return sqrt(a); }
 __device__ __forceinline__ float sq(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// x / 6 correctly rounded in three operations, for x = 0 or 2^-960 <= |x| <= 2^905 (Markstein's correction step with
+// a = RN(1/6) = (1/6)(1 - 2^-54)): x a is within ulp(x/6)/2 of x/6, so q0 = RN(x a) is within 1 ulp, r = 6 q0 - x is
+// an integer multiple of 2 ulp(x/6) smaller than 3 of them and exact in one FMA, and q0 - r a equals
+// x/6 + (q0 - x/6) 2^-54 before its one rounding. x/6 = (M/3) 2^e for an integer significand M is either
+// representable or M/3 leaves a remainder of 1/3 or 2/3, which puts x/6 at least 1/6 ulp from every midpoint; an
+// error below 2^-54 ulp cannot cross one, so the result is the IEEE quotient. Zeros keep their sign (r = +0, and
+// -r a + q0 = q0). Below the range the quotient may be subnormal and a midpoint (x = 9 2^-1074 gives 2^-1074, the
+// division 2 2^-1074); above it, or for non-finite x, the FMA can overflow or give NaN: callers keep the IEEE
+// division there (range_fits_quot6).
+__host__ __device__ __forceinline__ double quot6(double x) {
+    constexpr double a = 1.0 / 6.0;
+    const double q0 = x * a;
+    const double r = __builtin_fma(q0, 6.0, -x);
+    return __builtin_fma(-r, a, q0);
+}
+// v is 0 or 2^-900 <= |v| <= 2^900: the numerators of WENO5-Z's candidates (sums of 2, 7 and 11 times such values)
+// are then 0 or between 2^-952 and 2^905 in magnitude, inside the range of quot6. NaN is outside.
+__host__ __device__ __forceinline__ bool range_fits_quot6(double v) {
+    const double m = v < 0.0 ? -v : v;
+    return v == 0.0 || (m >= 0x1p-900 && m <= 0x1p+900);
+}
 // the absolute floor of the WENO regularisation: 1e-300 in FP64 (Schemes.cpp:331, 343); the smallest power of ten
 // above FLT_MIN in FP32, where 1e-300 would round to zero and 0/0 would reach the weights of an all-zero stencil
 template <typename T> __device__ __forceinline__ T weno_floor() { return T(1e-300); }
@@ -265,12 +287,19 @@ __device__ __forceinline__ T weno3z(T c0, T c1, T c2, T eps_rel) {
     return dv(a0 * v0 + a1 * v1, a0 + a1);
 }
 
-template <typename T>
+// the candidates' quotients by 6: the IEEE division, or quot6 (the same bits) when every stencil value is known to be
+// in range_fits_quot6 (Q6)
+template <bool Q6, typename T> __device__ __forceinline__ T by6(T x) {
+    if constexpr (Q6 && std::is_same<T, double>::value) return quot6(x);
+    else return dv(x, T(6.0));
+}
+
+template <typename T, bool Q6 = false>
 __device__ __forceinline__ T weno5z(T c0, T c1, T c2, T c3, T c4, T eps_rel) {
     const T eps = eps_rel * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4) * T(0.2) + weno_floor<T>();
-    const T p0 = dv(T(2.0) * c0 - T(7.0) * c1 + T(11.0) * c2, T(6.0));
-    const T p1 = dv(-c1 + T(5.0) * c2 + T(2.0) * c3, T(6.0));
-    const T p2 = dv(T(2.0) * c2 + T(5.0) * c3 - c4, T(6.0));
+    const T p0 = by6<Q6>(T(2.0) * c0 - T(7.0) * c1 + T(11.0) * c2);
+    const T p1 = by6<Q6>(-c1 + T(5.0) * c2 + T(2.0) * c3);
+    const T p2 = by6<Q6>(T(2.0) * c2 + T(5.0) * c3 - c4);
     const T s0 = c0 - T(2.0) * c1 + c2, t0 = c0 - T(4.0) * c1 + T(3.0) * c2;
     const T s1 = c1 - T(2.0) * c2 + c3, t1 = c1 - c3;
     const T s2 = c2 - T(2.0) * c3 + c4, t2 = T(3.0) * c2 - T(4.0) * c3 + c4;

@@ -23,6 +23,10 @@ struct DevState {
     long long reverted;
     int done, diverged, steps, max_steps;
     double ut[6];   // mass-flow inlet velocity per side of the current residual (k_massflow)
+    // nonzero: a primitive the residual's k_cells stored (FP64 planes) is outside range_fits_quot6, so the face
+    // kernels take the IEEE division for WENO5-Z's quotients by 6. k_cells only sets it; k_update, which ends every
+    // residual of the separate passes after its face kernels, clears it, and a run starts with it clear.
+    int wide;
 };
 
 // n / d for 0 <= n < 2^31 as (umulhi(n, m) + n) >> l, with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1

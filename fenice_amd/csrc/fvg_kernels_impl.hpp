@@ -415,8 +415,9 @@ __device__ __forceinline__ double cell_gradients_f32(const KP& p, long long c, i
 // beyond the other side; with a ghost slab (boundary_order = 2) it writes the primitives of its ghost state at its
 // mirror position; a side with neither has its halo layers copy the edge cell. u: the cell's state, w: its
 // primitives as stored. CONS: the planes Wo hold the conservative state (w = u, and a ghost is stored as it is).
+// wide: set when a ghost value stored is outside range_fits_quot6 (the copies of w are the caller's to check).
 template <int NV, typename T, bool CONS = false>
-__device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int k, const double* u, const T* w) {
+__device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int k, const double* u, const T* w, bool& wide) {
     auto put = [&](int d, int t, const T* v) {
         const long long pc = pidx_d(p, d, i, j, k, t);
 #pragma unroll
@@ -431,6 +432,10 @@ __device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int 
         } else {
             c2p<NV>(p.rs.eos, ug, wg, (double*)nullptr);
             to_t<NV>(wg, v);
+        }
+        if constexpr (!CONS && std::is_same<T, double>::value) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) wide = wide || !range_fits_quot6(v[q]);
         }
         put(d, t, v);
     };
@@ -462,6 +467,7 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
     const int i = rb.i, j = rb.j, k = rb.k;
     const long long c = ((long long)k * p.ny + j) * p.nx + i;
     double local = 0.0;
+    bool wide = false;   // a primitive stored in the FP64 planes is outside range_fits_quot6
     if (i < p.nx && !p.st->done) {
         const long long pc = pidx(p, i, j, k);
         if constexpr (std::is_same<T, float>::value) {
@@ -476,7 +482,7 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             float* Wo = (float*)p.W;
 #pragma unroll
             for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
-            push_halo<NV, float>(p, Wo, i, j, k, u, w);
+            push_halo<NV, float>(p, Wo, i, j, k, u, w, wide);
             if (p.viscous) local = smax(local, cell_gradients_f32<NV>(p, c, i, j, k, w));
         } else {
             double u[NV], w[NV];
@@ -485,17 +491,21 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             if (p.quad4) {   // fourth-order fluxes: the padded conservative state; the primitives only for the viscous term
 #pragma unroll
                 for (int q = 0; q < NV; ++q) p.Wc[q * p.npad + pc] = u[q];
-                push_halo<NV, double, true>(p, p.Wc, i, j, k, u, u);
+                push_halo<NV, double, true>(p, p.Wc, i, j, k, u, u, wide);
             }
             if (!p.quad4 || p.viscous) {
                 double* Wo = (double*)p.W;
 #pragma unroll
-                for (int q = 0; q < NV; ++q) Wo[q * p.npad + pc] = w[q];
-                push_halo<NV, double>(p, Wo, i, j, k, u, w);
+                for (int q = 0; q < NV; ++q) {
+                    Wo[q * p.npad + pc] = w[q];
+                    wide = wide || !range_fits_quot6(w[q]);
+                }
+                push_halo<NV, double>(p, Wo, i, j, k, u, w, wide);
             }
             if (p.viscous) local = smax(local, cell_gradients_f64<NV>(p, c, i, j, k, w));
         }
     }
+    if (wide) p.st->wide = 1;   // a plain store of the same value from every lane that found one
     if (p.viscous) block_max_slot(p, local, SPD_VISC);
 }
 
@@ -802,8 +812,9 @@ __device__ __forceinline__ void store_face(const KP& p, long long f, int i, int 
 
 // The two states of face (i, j, k) normal to D (t its coordinate along D, n the cells along D) that the Riemann
 // solver takes: reconstructed from the padded primitives, first-order at a side without a halo, or the
-// MUSCL-Hancock predicted states (Schemes.cpp:389-749, 2934-2988).
-template <int NV, int D, int MODE, typename T>
+// MUSCL-Hancock predicted states (Schemes.cpp:389-749, 2934-2988). Q6: WENO5-Z divides by 6 with quot6 (every
+// stencil value in range_fits_quot6; the same bits).
+template <int NV, int D, int MODE, typename T, bool Q6 = false>
 __device__ __forceinline__ void face_states(const KP& p, int i, int j, int k, int t, int n, T* UL, T* UR) {
     const bool lo = t == 0, hi = t == n;
     // the cells left and right of the face (their coordinate along D is t - 1 and t)
@@ -827,8 +838,8 @@ __device__ __forceinline__ void face_states(const KP& p, int i, int j, int k, in
                 for (int s = 0; s < 6; ++s) stencil_prim<NV, T>(p, D, i, j, k, t - 3 + s, c[s]);
 #pragma unroll
                 for (int q = 0; q < NV; ++q) {
-                    wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
-                    wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+                    wl[q] = weno5z<T, Q6>(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                    wr[q] = weno5z<T, Q6>(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
                 }
             } else {   // four cells t-2 .. t+1
                 T c[4][NV];
@@ -901,7 +912,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (D != 2) face_coords<NV, D>(p, f, i, j, k, t);
         const int n = ncell(p, D);
         T UL[NV], UR[NV];
-        face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
+        if constexpr (MODE == MODE_PLAIN && std::is_same<T, double>::value) {
+            // uniform over the launch: quot6 unless k_cells stored a primitive outside its range
+            if (__builtin_amdgcn_readfirstlane(p.st->wide) == 0) face_states<NV, D, MODE, T, true>(p, i, j, k, t, n, UL, UR);
+            else face_states<NV, D, MODE, T, false>(p, i, j, k, t, n, UL, UR);
+        } else {
+            face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
+        }
         T F[NV], sm;
         face_flux<NV>(p.rs, UL, UR, D, F, sm);
         if (double(sm) > local) local = double(sm);
@@ -1127,6 +1144,8 @@ template <int NV, int UPD, typename T = double>
 __global__ __launch_bounds__(256) void k_update(KP p, const double* __restrict__ U0, const double* __restrict__ Ut, double* __restrict__ out) {
     const RowBlock rb = row_block(p);
     const int i = rb.i, j = rb.j, k = rb.k;
+    // the residual's face kernels have read the range flag of k_cells: clear it for the next residual
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.st->wide = 0;
     if (i >= p.nx || p.st->done) return;
     const long long c = ((long long)k * p.ny + j) * p.nx + i;
     const long long fx = ((long long)k * p.ny + j) * (p.nx + 1) + i;

@@ -325,3 +325,68 @@ def test_monitor_row_matches_the_oracle(oracle, kind):
             np.sqrt((R[:, 3] ** 2).sum() / n)]
     assert np.all(np.abs(row - want) <= 1e-12 * np.abs(want)), (row, want)
     assert row[3] > 0 and row[4] > 0 and row[5] > 0
+
+
+def quot6_emulated(x):
+    """The face kernels' three-operation quotient by 6 (q0 = RN(x a), r = RN(6 q0 - x), RN(q0 - r a) with a = RN(1/6)),
+    each rounding done exactly here by rational arithmetic."""
+    from fractions import Fraction
+    a = 1.0 / 6.0
+    q0 = x * a
+    r = float(Fraction(q0) * 6 - Fraction(x))
+    return float(Fraction(q0) - Fraction(r) * Fraction(a))
+
+
+def tiny_velocity_state(nx, ny, nz, seed):
+    """rho = 1 and u, p of order one; v (and w in 3-D) subnormal multiples of 2^-1074, where quotients by 6 can be
+    ties that the three-operation form rounds the wrong way."""
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y = (i.ravel() + 0.5) / nx * 2 * np.pi, (j.ravel() + 0.5) / ny * 2 * np.pi
+    n = nx * ny * nz
+    tiny = 2.0 ** -1074
+    u = 0.3 * np.sin(y)
+    v = rng.integers(1, 60, n) * tiny
+    w = rng.integers(1, 60, n) * tiny if nz > 1 else np.zeros(n)
+    p = 1.0 + 0.1 * np.cos(x)
+    U = np.zeros((n, 5))
+    U[:, 0] = 1.0
+    U[:, 1], U[:, 2], U[:, 4] = u, v, w
+    U[:, 3] = p / 0.4 + 0.5 * (u * u + v * v + w * w)
+    return U
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 1), (8, 9, 7)], ids=["2d", "3d"])
+def test_weno5_quotients_outside_the_fast_range(oracle, dims):
+    # k_cells finds primitives below 2^-900 and the face kernels take the IEEE division for WENO5-Z's quotients by 6
+    # (Schemes.cpp:343-345); the state is chosen so that the three-operation quotient would differ on its stencils
+    nx, ny, nz = dims
+    say(f"weno5 quotients outside the fast range {dims}")
+    U = tiny_velocity_state(nx, ny, nz, 5)
+    v = U[:, 2].reshape(nz, ny, nx)
+    num = [2.0 * np.roll(v, 2, 2) - 7.0 * np.roll(v, 1, 2) + 11.0 * v, -np.roll(v, 1, 2) + 5.0 * v + 2.0 * np.roll(v, -1, 2),
+           2.0 * v + 5.0 * np.roll(v, -1, 2) - np.roll(v, -2, 2)]
+    differ = sum(float(quot6_emulated(float(x))) != float(x) / 6.0 for a in num for x in a.ravel())
+    assert differ > 0   # the fast form would give other bits here
+    keys = base_keys(nx, ny, nz, reconstruction="weno5")
+    cfg = fvg.config_from_keys(keys)
+    two_d = nz == 1
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        Rd, Sd = s.residual()
+        Ro, So = oracle.residual(cfg, U)
+        assert Sd == So and same_bits(Rd, Ro, two_d)
+        Uo = U.copy()
+        t, lag = 0.0, 0.0
+        for _ in range(3):
+            r = s.step(t, 10.0)
+            dto, so, lag, ok = oracle.step(cfg, Uo, t, 10.0, 0.0, lag)
+            assert ok and r.dt == dto and r.s_max == so
+            t += r.dt
+        assert same_bits(s.download(), Uo, two_d)
+        # a state in range afterwards takes the fast form again, with the same bits
+        U2 = smooth_state(nx, ny, nz, 6)
+        s.upload(U2)
+        Rd, Sd = s.residual()
+        Ro, So = oracle.residual(cfg, U2)
+        assert Sd == So and same_bits(Rd, Ro, two_d)
