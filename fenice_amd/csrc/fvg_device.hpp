@@ -53,7 +53,7 @@ __host__ __device__ __forceinline__ double quot6(double x) {
 // are then 0 or between 2^-952 and 2^905 in magnitude, inside the range of quot6. NaN is outside.
 __host__ __device__ __forceinline__ bool range_fits_quot6(double v) {
     const double m = v < 0.0 ? -v : v;
-    return v == 0.0 || (m >= 0x1p-900 && m <= 0x1p+900);
+    return (v == 0.0) | ((m >= 0x1p-900) & (m <= 0x1p+900));   // no short-circuit branches
 }
 // the absolute floor of the WENO regularisation: 1e-300 in FP64 (Schemes.cpp:331, 343); the smallest power of ten
 // above FLT_MIN in FP32, where 1e-300 would round to zero and 0/0 would reach the weights of an all-zero stencil

@@ -90,6 +90,9 @@ __global__ __launch_bounds__(kMassflowThreads) void k_massflow(KP p) {
 __global__ void k_control(KP p, int op, int arg) {
     const int scheme = arg;
     DevState* s = p.st;
+    // every control launch comes after a residual's face kernels or between steps, never between a k_cells and its
+    // faces: clear the quotient range flag for the next k_cells (the fused passes have no k_update to clear it)
+    if (threadIdx.x == 0) s->wide = 0;
     if (!s->done) {
         const int l = threadIdx.x;
         unsigned long long* cs = p.spd + (size_t)(SPD_CONV * kSpeedSlots + l) * kSlotStride;

@@ -24,8 +24,9 @@ struct DevState {
     int done, diverged, steps, max_steps;
     double ut[6];   // mass-flow inlet velocity per side of the current residual (k_massflow)
     // nonzero: a primitive the residual's k_cells stored (FP64 planes) is outside range_fits_quot6, so the face
-    // kernels take the IEEE division for WENO5-Z's quotients by 6. k_cells only sets it; k_update, which ends every
-    // residual of the separate passes after its face kernels, clears it, and a run starts with it clear.
+    // kernels take the IEEE division for WENO5-Z's quotients by 6. k_cells only sets it; k_update (the end of every
+    // residual of the separate passes) and k_control (after a residual's faces or between steps) clear it, never
+    // between a k_cells and the face kernels that read its planes.
     int wide;
 };
 

@@ -435,7 +435,7 @@ __device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int 
         }
         if constexpr (!CONS && std::is_same<T, double>::value) {
 #pragma unroll
-            for (int q = 0; q < NV; ++q) wide = wide || !range_fits_quot6(v[q]);
+            for (int q = 0; q < NV; ++q) wide |= !range_fits_quot6(v[q]);
         }
         put(d, t, v);
     };
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
 #pragma unroll
                 for (int q = 0; q < NV; ++q) {
                     Wo[q * p.npad + pc] = w[q];
-                    wide = wide || !range_fits_quot6(w[q]);
+                    wide |= !range_fits_quot6(w[q]);
                 }
                 push_halo<NV, double>(p, Wo, i, j, k, u, w, wide);
             }
@@ -946,7 +946,8 @@ __device__ __forceinline__ bool face_upgrade(const KP& p, int t) {
 template <int NV, int D> __device__ __forceinline__ unsigned char* kept_of(const KP& p) {
     return D == 0 ? p.ind_kept[0] : (D == 1 ? p.ind_kept[1] : p.ind_kept[2]);
 }
-template <int NV, int D, int USE, typename T>
+// Q6: the troubled faces' WENO5-Z divides by 6 with quot6 (every stencil value in range_fits_quot6; the same bits)
+template <int NV, int D, int USE, typename T, bool Q6 = false>
 __device__ __forceinline__ void learned_face(const KP& p, bool active, long long f, int i, int j, int k, int t, int n, T* F,
                                              T& sm) {
     const bool upgrade = active && face_upgrade<NV, D>(p, t);
@@ -1001,8 +1002,8 @@ __device__ __forceinline__ void learned_face(const KP& p, bool active, long long
         } else {
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
-                wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
-                wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+                wl[q] = weno5z<T, Q6>(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                wr[q] = weno5z<T, Q6>(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
             }
         }
         p2c<NV>(p.rs.eos, wl, UL);
@@ -1040,7 +1041,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     const int n = ncell(p, D);
     T F[NV], sm = T(0);
-    learned_face<NV, D, USE, T>(p, active, f, i, j, k, t, n, F, sm);
+    // uniform over the launch: quot6 for FP64 unless k_cells stored a primitive outside its range
+    if (std::is_same<T, double>::value && __builtin_amdgcn_readfirstlane(p.st->wide) == 0)
+        learned_face<NV, D, USE, T, std::is_same<T, double>::value>(p, active, f, i, j, k, t, n, F, sm);
+    else
+        learned_face<NV, D, USE, T>(p, active, f, i, j, k, t, n, F, sm);
     double local = 0.0;
     if (active) {
         if (double(sm) > local) local = double(sm);
@@ -1233,7 +1238,12 @@ __device__ __forceinline__ void face_all(const KP& p, bool active, long long f, 
     } else {
         if (!active) return;
         T UL[NV], UR[NV];
-        face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
+        if constexpr (MODE == MODE_PLAIN && std::is_same<T, double>::value) {
+            if (__builtin_amdgcn_readfirstlane(p.st->wide) == 0) face_states<NV, D, MODE, T, true>(p, i, j, k, t, n, UL, UR);
+            else face_states<NV, D, MODE, T, false>(p, i, j, k, t, n, UL, UR);
+        } else {
+            face_states<NV, D, MODE, T>(p, i, j, k, t, n, UL, UR);
+        }
         face_flux<NV>(p.rs, UL, UR, D, Fc, sm);
     }
     if (!active) return;
