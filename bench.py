@@ -31,12 +31,12 @@ BYTES_PER_CELL_STAGE_RK3_3D = 8.0 * 5 * (2 + 3 + 3) / 3.0   # 106.7 B: stage inp
 # three SSP-RK3 stages), from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a 1 GiB
 # device copy reads in the same runs); the runs and per-kernel figures are in the profile files named here. These
 # are figures of those counter runs, not of the run that prints them.
-MEASURED_STAGE_TRAFFIC = {("fp64", 256): (16.776e9, "profiles/tgv256_weno5_rk3_fp64.md"),
+MEASURED_STAGE_TRAFFIC = {("fp64", 256): (16.888e9, "profiles/tgv256_weno5_rk3_fp64.md"),
                           ("fp32", 256): (7.678e9, "profiles/tgv256_weno5_rk3_fp32.md")}
 # VALU instructions per wavefront (SQ_INSTS_VALU / SQ_WAVES, one face or cell per lane) of each kernel of the
 # workload, from the same counter runs; with this run's event time of the kernel they give its VALU issue rate. A
 # wave64 VALU instruction occupies a SIMD for 4 cycles; MI355X has 1,024 SIMDs at up to 2.4 GHz.
-VALU_PER_WAVE = {"fp64": ({"k_cells": 552, "k_faces_x": 2137, "k_faces_y": 2141, "k_faces_z": 2110}, "profiles/tgv256_weno5_rk3_fp64.md")}
+VALU_PER_WAVE = {"fp64": ({"k_cells": 578, "k_faces_x": 1893, "k_faces_y": 1902, "k_faces_z": 1873}, "profiles/tgv256_weno5_rk3_fp64.md")}
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 from fenice_amd.workloads import taylor_green as tgv_state  # noqa: E402
