@@ -331,7 +331,8 @@ def other_modes(U0, n, steps, weights):
 
 def side_workload(name: str) -> dict:
     """Throughput of the other configurations of the path (parity cases, not the headline): the 1-D ensemble
-    (1024 Riemann problems of 128..8192 cells to t = 0.15, one launch after the autotuner), the isentropic vortex at
+    (1024 Riemann problems of 128..8192 cells to t = 0.15, one launch after the autotuner, the largest instances on
+    teams of workgroups), the isentropic vortex at
     256^2 under MUSCL-Hancock (CFL 0.7, 200 steps, graph replay), and the Kelvin-Helmholtz layer at 2048^2
     (WENO5-Z + SSP-RK3, 30 steps), the 512^3 Taylor-Green vortex in its three precision modes (with the kinetic energy,
     its dissipation and the density error against FP64 every 100 steps), and the fourth-order flux quadrature."""
@@ -343,6 +344,7 @@ def side_workload(name: str) -> dict:
             e.upload(U)
             block, _ = e.tune(0.0075, 400000)
             r = e.run(0.15, 400000)
+            plan = e.plan
         stages = 2.0 * float((n.astype(np.float64) * r.steps).sum())
         # beside it, 2-D instances: 256 isentropic vortices on square periodic meshes of 32..256 cells a side, MUSCL
         # (Van Leer) + HLLC + SSP-RK2, CFL 0.5, to t = 1 (the vortex moves by one unit of its 10-wide box)
@@ -355,6 +357,7 @@ def side_workload(name: str) -> dict:
         vstages = 2.0 * float(((nv.astype(np.float64) ** 2) * rv.steps).sum())
         return {"workload": "1024 1-D Riemann problems, 128..8192 cells (4,243,015), MUSCL MC + HLLC + SSP-RK2, CFL 0.5, "
                             "to t = 0.15", "block": block, "kernel_ms": round(r.ms, 2),
+                "teams": {"teams": plan[0], "workgroups_in_teams": plan[1], "workgroups": plan[2]},
                 "value": round(stages / (r.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                 "instances_done": int((r.t >= 0.15).sum()), "diverged": int(r.diverged.sum()),
                 "steps_max": int(r.steps.max()), "steps_total": int(r.steps.sum()),

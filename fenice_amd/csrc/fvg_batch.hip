@@ -41,6 +41,22 @@ struct BOut {
     int steps, diverged;
 };
 
+// A team: m workgroups that run one instance together, each over its own chunk of cells (rank r: cells
+// [n r / m, n (r + 1) / m)), exchanging the primitives of the KH cells next to their chunk and their wave-speed
+// maxima through a TeamSync in global memory, with two team barriers per stage.
+constexpr int KH = 3;   // stencil reach beyond a chunk: the WENO5-Z faces of a chunk read 3 cells on each side
+struct BTeam {
+    int inst, wg0, m, pad;   // instance, first workgroup, workgroups
+};
+struct TeamSync {
+    unsigned long long count;    // barrier arrivals so far; barrier k of the run completes at (k + 1) m
+    int broken;                  // a member gave up waiting (bounded spin): every member leaves the instance
+    int pad;
+    double smax[16];             // per rank: the wave-speed maximum of its chunk (the barrier after the faces)
+    double halo[16][2][KH * 3];  // per rank, low and high end of its chunk: (rho, u, p) of its KH end cells
+};
+constexpr int kMaxTeam = 16;
+
 struct BP {
     RiemannParams rs;
     BcParams bp;
@@ -52,8 +68,15 @@ struct BP {
     double* W;          // 3 planes (rho, u, p): primitives of the stage input
     double* F;          // 3 planes: x-face fluxes
     const BInst* inst;
-    const int* order;   // workgroup -> instance (longest estimated run first)
+    const int* order;   // workgroup -> instance (longest estimated run first) when queue is null
     BOut* out;
+    // teams and queue (a cooperative launch of resident workgroups; null queue: one workgroup per instance)
+    const BTeam* teams;
+    int nteams, team_wgs;     // workgroups [0, team_wgs) belong to the teams, in team order
+    const int* queue;         // the other instances, longest first, taken one at a time by any workgroup
+    int nqueue, pad;
+    unsigned* qnext;          // next queue entry (zeroed before the launch)
+    TeamSync* sync;           // one per team (zeroed before the launch)
 };
 
 // the state of cell i of the instance as the 2-D path holds it: (rho, rho u, rho v = +0, E)
@@ -94,6 +117,35 @@ __device__ __forceinline__ void bstencil(const BP& b, const BInst& in, const dou
     }
 }
 
+// The cells of a workgroup's chunk [lo, hi) of the instance (the whole instance for a workgroup of its own), and
+// the primitives (rho, u, p) of the KH cells below and above it that other members of its team hold (has_lo /
+// has_hi; across the ends of a periodic instance the cells of the other end).
+struct Chunk {
+    int lo, hi;
+    bool has_lo, has_hi;
+    const double* hl;   // cells lo - KH .. lo - 1
+    const double* hh;   // cells hi .. hi + KH - 1
+};
+
+// the padded primitive state at x index t as the stencils of the chunk's faces read it
+__device__ __forceinline__ void cstencil(const BP& b, const BInst& in, const double* Us, const Chunk& ch, int t, double* w) {
+    if (t >= ch.lo && t < ch.hi) {
+        bload_prim(b, in, t, w);
+        return;
+    }
+    const double* h = nullptr;
+    if (t < ch.lo && ch.has_lo) h = ch.hl + (t - ch.lo + KH) * 3;
+    else if (t >= ch.hi && ch.has_hi) h = ch.hh + (t - ch.hi) * 3;
+    if (h) {
+        w[0] = h[0];
+        w[1] = h[1];
+        w[2] = 0.0;
+        w[3] = h[2];
+        return;
+    }
+    bstencil(b, in, Us, t, w);
+}
+
 // the first-order state beyond x side `side` of its boundary cell (outside_first_order)
 __device__ __forceinline__ void boutside(const BP& b, const BInst& in, const double* Us, int side, double* ug) {
     if (bc_of(b.bp, side) == 7) {
@@ -108,7 +160,7 @@ __device__ __forceinline__ void boutside(const BP& b, const BInst& in, const dou
 // face f (between cells f - 1 and f) of the instance: the reconstruction (RECON = fvg_recon) and Riemann solve
 // of the face pass
 template <int RECON>
-__device__ __forceinline__ void bface(const BP& b, const BInst& in, const double* Us, int f, double* F, double& sm) {
+__device__ __forceinline__ void bface(const BP& b, const BInst& in, const double* Us, const Chunk& ch, int f, double* F, double& sm) {
     const int n = in.n;
     const bool lo = f == 0, hi = f == n;
     double UL[4], UR[4];
@@ -123,7 +175,7 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
         if constexpr (RECON == 3) {
             double c[6][4];
 #pragma unroll
-            for (int s = 0; s < 6; ++s) bstencil(b, in, Us, f - 3 + s, c[s]);
+            for (int s = 0; s < 6; ++s) cstencil(b, in, Us, ch, f - 3 + s, c[s]);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
@@ -132,7 +184,7 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
         } else {
             double c[4][4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) bstencil(b, in, Us, f - 2 + s, c[s]);
+            for (int s = 0; s < 4; ++s) cstencil(b, in, Us, ch, f - 2 + s, c[s]);
             if constexpr (RECON == 2) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -191,13 +243,66 @@ __device__ __forceinline__ double bmax_all(double v, double* red) {
     return m;
 }
 
+// Barrier of the m workgroups of a team (thread 0 of each arrives, the workgroup waits on its own barrier around
+// it). Arrivals only ever increase ts->count; barrier k of the run completes when it reaches (k + 1) m. The wait is
+// bounded: a member that waits longer than kTeamSpin polls, or sees another member give up, marks the team broken
+// and returns false, and the whole team leaves the instance (fvg_batch_run reports the failure).
+constexpr unsigned kTeamSpin = 1u << 22;
+// a double exchanged between the members of a team: device-scope atomic store and load of its bits
+__device__ __forceinline__ void team_put(double* p, double v) {
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double team_get(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ bool team_barrier(TeamSync* ts, int m, unsigned long long& k, int* flag) {
+    __syncthreads();   // the workgroup's writes of this phase precede thread 0's arrival
+    if (threadIdx.x == 0) {
+        const unsigned long long target = (k + 1) * (unsigned long long)m;
+        int ok = 1;
+        __hip_atomic_fetch_add(&ts->count, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(&ts->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&ts->broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > kTeamSpin) {
+                __hip_atomic_store(&ts->broken, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the other members' writes before their arrival
+        *flag = ok;
+    }
+    ++k;
+    __syncthreads();
+    return *flag != 0;
+}
+
+// The run of instance id by rank r of a team of m workgroups (m = 1: the workgroup alone, ts unused): steps and
+// stages of the time scheme over the rank's chunk of cells. With m > 1 each stage has two team barriers: after the
+// primitives (the ranks publish the primitives of their KH end cells and read their neighbours'), and after the
+// faces (the ranks publish their chunk's wave-speed maximum and take the team's). The maximum of the same values in
+// another order is the same double, and every rank computes the same dt, time and step count, so the ranks leave
+// the loop together and the instance takes the bits of a single workgroup's run. A chunk's faces are lo .. hi; face
+// hi, which the next rank owns, is computed again for the chunk's last cell and kept in LDS.
 template <int BS, int RECON>
-__global__ __launch_bounds__(BS) void k_ensemble(BP b) {
-    __shared__ double red[BS / 64];
-    const int id = b.order[blockIdx.x];
+__device__ void run_instance(const BP& b, int id, int r, int m, TeamSync* ts, double* red, double* hl, double* hh, double* fx,
+                             int* flag) {
     const BInst in = b.inst[id];
     const int n = in.n;
     const int nst = b.scheme + 1;
+    const bool team = m > 1;
+    const bool per = in.halo_lo == 1;   // periodic: the first rank's low neighbour is the last rank
+    Chunk ch;
+    ch.lo = (int)((long long)n * r / m);
+    ch.hi = (int)((long long)n * (r + 1) / m);
+    ch.has_lo = team && (r > 0 || per);
+    ch.has_hi = team && (r < m - 1 || per);
+    ch.hl = hl;
+    ch.hh = hh;
+    const bool extra = team && r < m - 1;   // face hi belongs to the next rank
+    const int r_lo = (r + m - 1) % m, r_hi = (r + 1) % m;
+    unsigned long long nbar = 0;
     double t = 0.0, dt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s_step = 0.0;
     int steps = 0, diverged = 0;
     while (t < b.t_end && steps < b.max_steps) {
@@ -205,7 +310,7 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
             const double* Us = st == 1 ? b.U : b.Ut;
             // primitives of the stage input (k_cells) and the wave speed of each cell's y faces
             double local = 0.0;
-            for (int i = threadIdx.x; i < n; i += BS) {
+            for (int i = ch.lo + (int)threadIdx.x; i < ch.hi; i += BS) {
                 double u[4], w[4];
                 bload_cons(b, in, Us, i, u);
                 c2p<4>(b.rs.eos, u, w, (double*)nullptr);
@@ -216,18 +321,64 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
                 const double sy = by_speed<RECON>(b, in, Us, i);
                 if (sy > local) local = sy;
             }
+            if (team) {
+                // lanes of the first wavefront publish the chunk's end cells (KH cells x (rho, u, p)) one value
+                // each; the barrier's workgroup barrier and thread 0's release order them before its arrival. After
+                // it the same lanes read the neighbours' into LDS, all loads in flight at once: the low halo is the
+                // low neighbour's high end, the high halo the high neighbour's low end.
+                __syncthreads();
+                if (threadIdx.x < 2 * KH * 3) {
+                    const int side = threadIdx.x / (KH * 3), e = threadIdx.x % (KH * 3);
+                    const long long c = in.c0 + (side ? ch.hi - KH : ch.lo) + e / 3;
+                    team_put(&ts->halo[r][side][e], b.W[(long long)(e % 3) * b.nc + c]);
+                }
+                if (!team_barrier(ts, m, nbar, flag)) {
+                    diverged = 2;
+                    break;
+                }
+                if (threadIdx.x < 2 * KH * 3) {
+                    const int side = threadIdx.x / (KH * 3), e = threadIdx.x % (KH * 3);
+                    (side ? hh : hl)[e] = team_get(&ts->halo[side ? r_hi : r_lo][side ? 0 : 1][e]);
+                }
+            }
             __syncthreads();
             // x faces (k_faces)
-            for (int f = threadIdx.x; f <= n; f += BS) {
+            for (int f = ch.lo + (int)threadIdx.x; f <= ch.hi; f += BS) {
                 double F[4], sm;
-                bface<RECON>(b, in, Us, f, F, sm);
+                bface<RECON>(b, in, Us, ch, f, F, sm);
                 if (sm > local) local = sm;
-                const long long x = in.f0 + f;
-                b.F[x] = F[0];
-                b.F[b.nf + x] = F[1];
-                b.F[2 * b.nf + x] = F[3];
+                if (extra && f == ch.hi) {
+                    fx[0] = F[0];
+                    fx[1] = F[1];
+                    fx[2] = F[3];
+                } else {
+                    const long long x = in.f0 + f;
+                    b.F[x] = F[0];
+                    b.F[b.nf + x] = F[1];
+                    b.F[2 * b.nf + x] = F[3];
+                }
             }
-            const double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            if (team) {   // the team's maximum
+                if (threadIdx.x == 0) team_put(&ts->smax[r], S);
+                if (!team_barrier(ts, m, nbar, flag)) {
+                    diverged = 2;
+                    break;
+                }
+                if (threadIdx.x < 64) {   // lane k reads rank k's maximum; a NaN of any rank is kept, so that the
+                                          // team reports the divergence
+                    double v = threadIdx.x < (unsigned)m ? team_get(&ts->smax[threadIdx.x]) : 0.0;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const double w = __shfl_xor(v, o, 64);
+                        v = (v < w || w != w) ? w : v;
+                    }
+                    if (threadIdx.x == 0) red[0] = v;
+                }
+                __syncthreads();
+                S = red[0];
+                __syncthreads();
+            }
             // the scalar part of the scheme (k_control)
             if (st == 1) {
                 s1 = S;
@@ -239,12 +390,13 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
                 s3 = S;
             }
             // balance and stage combination (k_update)
-            for (int i = threadIdx.x; i < n; i += BS) {
+            for (int i = ch.lo + (int)threadIdx.x; i < ch.hi; i += BS) {
                 const long long c = in.c0 + i, x = in.f0 + i;
+                const bool last = extra && i == ch.hi - 1;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const double* Fq = b.F + q * b.nf;
-                    const double fl = Fq[x], fr = Fq[x + 1];
+                    const double fl = Fq[x], fr = last ? fx[q] : Fq[x + 1];
                     const double R = (0.0 - (fr * in.Ax - fl * in.Ax)) * in.inv_vol;
                     const long long y = q * b.nc + c;
                     if (b.scheme == 0) {
@@ -262,6 +414,7 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
             }
             __syncthreads();
         }
+        if (diverged == 2) break;
         s_step = b.scheme == 0 ? s1 : (b.scheme == 1 ? smax(s1, s2) : smax(s1, smax(s2, s3)));
         steps += 1;
         // check_progress (SimulationInternal.hpp:259-267)
@@ -271,7 +424,7 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
         }
         t += dt;
     }
-    if (threadIdx.x == 0) {
+    if (r == 0 && threadIdx.x == 0) {
         BOut o;
         o.t = t;
         o.s_last = s_step;
@@ -279,6 +432,43 @@ __global__ __launch_bounds__(BS) void k_ensemble(BP b) {
         o.steps = steps;
         o.diverged = diverged;
         b.out[id] = o;
+    }
+    __syncthreads();
+}
+
+// Workgroups [0, team_wgs) run the teams' instances, then every workgroup takes the queued instances one at a time
+// (longest first) until none is left. Without a queue, workgroup k runs instance order[k] alone.
+template <int BS, int RECON>
+__global__ __launch_bounds__(BS) void k_ensemble(BP b) {
+    __shared__ double red[BS / 64];
+    __shared__ double hl[KH * 3], hh[KH * 3], fx[3];
+    __shared__ int flag, slot;
+    const int wg = blockIdx.x;
+    // 0: the launch's own instance (no queue), 1: the workgroup's team, 2: the queue
+    int phase = !b.queue ? 0 : (wg < b.team_wgs ? 1 : 2);
+    for (;;) {
+        int id, r = 0, m = 1;
+        TeamSync* ts = nullptr;
+        if (phase == 0) {
+            id = b.order[wg];
+        } else if (phase == 1) {
+            int k = 0;
+            while (wg >= b.teams[k].wg0 + b.teams[k].m) ++k;
+            id = b.teams[k].inst;
+            r = wg - b.teams[k].wg0;
+            m = b.teams[k].m;
+            ts = b.sync + k;
+        } else {
+            if (threadIdx.x == 0) slot = (int)atomicAdd(b.qnext, 1u);
+            __syncthreads();
+            const int k = slot;
+            __syncthreads();
+            if (k >= b.nqueue) break;
+            id = b.queue[k];
+        }
+        run_instance<BS, RECON>(b, id, r, m, ts, red, hl, hh, fx, &flag);
+        if (phase == 0) break;
+        phase = 2;
     }
 }
 
@@ -527,6 +717,15 @@ struct fvg_batch {
     bool uploaded = false;
     double last_ms = 0.0;
     std::string err;
+    // teams (1-D instances): the largest instances split over several workgroups of a cooperative launch
+    int max_team = 8, min_chunk = 512, force_teams = 0;
+    bool coop = false;                   // the device supports cooperative launches
+    std::vector<double> cost;            // estimated work per instance (N^2 x initial wave speed)
+    BTeam* d_teams = nullptr;
+    int* d_queue = nullptr;
+    unsigned* d_qnext = nullptr;
+    TeamSync* d_sync = nullptr;
+    int plan_teams = 0, plan_team_wgs = 0, plan_grid = 0;   // the last launch's plan
 };
 
 namespace {
@@ -554,6 +753,10 @@ void free_batch(fvg_batch* b) {
     hipFree(b->W);
     hipFree(b->F);
     hipFree(b->Usave);
+    hipFree(b->d_teams);
+    hipFree(b->d_queue);
+    hipFree(b->d_qnext);
+    hipFree(b->d_sync);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -582,7 +785,137 @@ void launch_bs(fvg_batch* b, const BP& p) {
     }
 }
 
+// The teams of a launch of `grid` resident workgroups of `bs` threads. With force_teams every instance of at least
+// 2 min_chunk cells gets min(max_team, N / min_chunk) workgroups while they last. Otherwise the plan shortens the run
+// greedily: an instance's run is modelled as steps x (a + b N / m + c [m > 1]) microseconds, with steps proportional
+// to its estimated work over N (work = N^2 x initial wave speed), a the fixed latency of a stage, b the time per cell
+// of a chunk and c the cost of the two team barriers of a stage (about 9, 0.0065 and 6 us measured on MI355X with
+// 1,024 threads; b grows as the square root of 1024 / bs). The longest modelled run takes one more workgroup while
+// that shortens it without raising the bound max(longest run, total workgroup time / grid), each chunk keeps at least min_chunk cells, a
+// team at most max_team workgroups and the teams at most three quarters of the grid. The other instances form the
+// queue, longest first.
+void plan_teams(const fvg_batch* b, int grid, int bs, std::vector<BTeam>& teams, std::vector<int>& queue) {
+    teams.clear();
+    queue.clear();
+    const int ninst = (int)b->order.size();
+    std::vector<int> m(b->n.size(), 1);
+    auto cap = [&](int id) {   // most workgroups instance id may take
+        const int n = b->n[id];
+        if (b->max_team < 2 || b->min_chunk < KH || n < 2 * b->min_chunk) return 1;
+        return std::min(b->max_team, std::min(kMaxTeam, n / b->min_chunk));
+    };
+    int wgs = 0;
+    if (b->force_teams) {
+        for (int id : b->order) {
+            const int k = cap(id);
+            if (k >= 2 && wgs + k <= grid) {
+                m[id] = k;
+                wgs += k;
+            }
+        }
+    } else if (b->max_team > 1 && grid > 1) {
+        const double A = 9.0, Bc = 0.0065 * std::sqrt(1024.0 / bs), C = 6.0;
+        auto run = [&](int id, int k) {
+            const double n = b->n[id], steps = b->cost[id] / std::max(1.0, n);
+            return steps * (A + Bc * n / k + (k > 1 ? C : 0.0));
+        };
+        double total = 0.0;
+        std::vector<std::pair<double, int>> heap;   // (modelled run, instance), longest on top
+        for (int id = 0; id < ninst; ++id) {
+            const double t = run(id, 1);
+            total += t;
+            heap.push_back({t, id});
+        }
+        std::make_heap(heap.begin(), heap.end());
+        const int max_wgs = grid * 3 / 4;
+        while (!heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end());
+            const auto [t, id] = heap.back();
+            heap.pop_back();
+            const int k = m[id] + 1, extra = (m[id] == 1 ? 2 : 1);
+            if (k > cap(id) || wgs + extra > max_wgs) break;
+            const double t2 = run(id, k);
+            const double total2 = total - m[id] * t + k * t2;
+            const double next = heap.empty() ? 0.0 : heap.front().first;
+            // taken while the bound does not rise (equal runs: the others follow) and this run gets shorter
+            if (!(t2 < t) || std::max(std::max(next, t2), total2 / grid) > std::max(t, total / grid)) break;
+            wgs += extra;
+            m[id] = k;
+            total = total2;
+            heap.push_back({t2, id});
+            std::push_heap(heap.begin(), heap.end());
+        }
+    }
+    wgs = 0;
+    for (int id : b->order) {
+        if (m[id] >= 2) {
+            teams.push_back({id, wgs, m[id], 0});
+            wgs += m[id];
+        } else {
+            queue.push_back(id);
+        }
+    }
+}
+
+// The teamed launch of the 1-D kernel: workgroups resident at once (a cooperative launch), teams then the queue.
+template <int BS, int RECON>
+int launch_teams(fvg_batch* b, BP p) {
+    int per_cu = 0, dev_cus = 0;
+    BCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ensemble<BS, RECON>, BS, 0));
+    BCHK(b, hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const int resident = per_cu * dev_cus;
+    if (resident < 1) return bset(b, FVG_EHIP, "the ensemble kernel cannot be resident");
+    std::vector<BTeam> teams;
+    std::vector<int> queue;
+    plan_teams(b, resident, BS, teams, queue);
+    int wgs = 0;
+    for (const BTeam& t : teams) wgs += t.m;
+    const int grid = std::min(resident, wgs + (int)queue.size());
+    if (!teams.empty()) BCHK(b, hipMemcpy(b->d_teams, teams.data(), sizeof(BTeam) * teams.size(), hipMemcpyHostToDevice));
+    if (!queue.empty()) BCHK(b, hipMemcpy(b->d_queue, queue.data(), sizeof(int) * queue.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemsetAsync(b->d_qnext, 0, sizeof(unsigned), b->stream));
+    if (!teams.empty()) BCHK(b, hipMemsetAsync(b->d_sync, 0, sizeof(TeamSync) * teams.size(), b->stream));
+    p.teams = b->d_teams;
+    p.nteams = (int)teams.size();
+    p.team_wgs = wgs;
+    p.queue = b->d_queue;
+    p.nqueue = (int)queue.size();
+    p.qnext = b->d_qnext;
+    p.sync = b->d_sync;
+    void* args[] = {&p};
+    BCHK(b, hipLaunchCooperativeKernel((const void*)k_ensemble<BS, RECON>, dim3(grid), dim3(BS), args, 0, b->stream));
+    b->plan_teams = (int)teams.size();
+    b->plan_team_wgs = wgs;
+    b->plan_grid = grid;
+    return FVG_OK;
+}
+
+template <int BS>
+int launch_teams_bs(fvg_batch* b, const BP& p) {
+    switch (p.recon) {
+    case 1: return launch_teams<BS, 1>(b, p);
+    case 2: return launch_teams<BS, 2>(b, p);
+    default: return launch_teams<BS, 3>(b, p);
+    }
+}
+
+// whether a launch of the 1-D kernel takes teams: cooperative launches available, a reconstruction (first-order
+// faces read the conservative state of the next chunk, which the ranks do not exchange) and max_team > 1
+bool teamed(const fvg_batch* b, const BP& p) { return b->dim == 1 && b->coop && b->max_team > 1 && p.recon != 0; }
+
 int launch_ensemble(fvg_batch* b, const BP& p, int block) {
+    if (teamed(b, p)) {
+        switch (block) {
+        case 64: return launch_teams_bs<64>(b, p);
+        case 128: return launch_teams_bs<128>(b, p);
+        case 256: return launch_teams_bs<256>(b, p);
+        case 512: return launch_teams_bs<512>(b, p);
+        default: return launch_teams_bs<1024>(b, p);
+        }
+    }
+    b->plan_teams = 0;
+    b->plan_team_wgs = 0;
+    b->plan_grid = b->ninst;
     switch (block) {
     case 64: launch_bs<64>(b, p); break;
     case 128: launch_bs<128>(b, p); break;
@@ -710,8 +1043,13 @@ int fvg_batch_create(const fvg_config* cfg_in, int ninst, const int* ncells, int
     if (hipMalloc(&b->U, cb) != hipSuccess || hipMalloc(&b->Ut, cb) != hipSuccess || hipMalloc(&b->W, cb) != hipSuccess ||
         hipMalloc(&b->Usave, cb) != hipSuccess || hipMalloc(&b->F, fb) != hipSuccess ||
         hipMalloc(&b->d_inst, sizeof(BInst) * ninst) != hipSuccess || hipMalloc(&b->d_order, sizeof(int) * ninst) != hipSuccess ||
-        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess)
+        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess || hipMalloc(&b->d_teams, sizeof(BTeam) * ninst) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(int) * ninst) != hipSuccess || hipMalloc(&b->d_qnext, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&b->d_sync, sizeof(TeamSync) * ninst) != hipSuccess)
         return fail(FVG_ENOMEM, "hipMalloc of the ensemble buffers");
+    int coop = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) == hipSuccess) b->coop = coop != 0;
+    b->cost.assign(ninst, 0.0);
     if (hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_order, b->order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice) != hipSuccess)
         return fail(FVG_EHIP, "hipMemcpy of the instance table");
@@ -876,6 +1214,7 @@ int fvg_batch_upload(fvg_batch* b, const double* U, long long ncells, int ncomp)
         cost[i] = (double)nc_i * lmax * (smax0 + 1e-3);
     }
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    b->cost = cost;
     BCHK(b, hipSetDevice(b->device));
     BCHK(b, hipMemcpy(b->U, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->ninst, hipMemcpyHostToDevice));
@@ -924,6 +1263,9 @@ int fvg_batch_run(fvg_batch* b, double t_end, double dt_max, int max_steps, doub
     if (ms) *ms = m;
     std::vector<BOut> o(b->ninst);
     BCHK(b, hipMemcpy(o.data(), b->d_out, sizeof(BOut) * b->ninst, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ninst; ++i)
+        if (o[i].diverged == 2)
+            return bset(b, FVG_EHIP, "instance " + std::to_string(i) + ": a workgroup of its team timed out at a team barrier");
     for (int i = 0; i < b->ninst; ++i) {
         if (t) t[i] = o[i].t;
         if (steps) steps[i] = o[i].steps;
@@ -941,6 +1283,24 @@ int fvg_batch_set_block(fvg_batch* b, int block) {
             return FVG_OK;
         }
     return bset(b, FVG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+}
+
+int fvg_batch_set_teams(fvg_batch* b, int max_team, int min_chunk, int force) {
+    if (!b) return bset(b, FVG_EINVAL, "null argument");
+    if (max_team < 1 || max_team > kMaxTeam) return bset(b, FVG_EINVAL, "max_team must be in 1 .. 16");
+    if (min_chunk < KH) return bset(b, FVG_EINVAL, "min_chunk must be at least 3 cells");
+    b->max_team = max_team;
+    b->min_chunk = min_chunk;
+    b->force_teams = force ? 1 : 0;
+    return FVG_OK;
+}
+
+int fvg_batch_get_plan(const fvg_batch* b, int* nteams, int* team_wgs, int* grid) {
+    if (!b) return FVG_EINVAL;
+    if (nteams) *nteams = b->plan_teams;
+    if (team_wgs) *team_wgs = b->plan_team_wgs;
+    if (grid) *grid = b->plan_grid;
+    return FVG_OK;
 }
 
 int fvg_batch_get_block(const fvg_batch* b, int* block) {
@@ -989,6 +1349,13 @@ int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cach
             best = k;
         }
     }
+    // the probe's step counts measure each instance's work better than the initial wave speed does (its waves form
+    // within the first steps): the queue order and the teams of later runs take N x probe steps as the work
+    std::vector<BOut> o(b->ninst);
+    BCHK(b, hipMemcpy(o.data(), b->d_out, sizeof(BOut) * b->ninst, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ninst; ++i) b->cost[i] = (double)b->n[i] * std::max(1, o[i].steps);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return b->cost[a] > b->cost[c]; });
+    BCHK(b, hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->ninst, hipMemcpyHostToDevice));
     BCHK(b, hipMemcpyAsync(b->U, b->Usave, cb, hipMemcpyDeviceToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     b->block = best;

@@ -32,7 +32,7 @@ EXPORTS = [
     "fvg_indicator_set", "fvg_indicator_stats", "fvg_indicator_samples", "fvg_trainer_create", "fvg_trainer_destroy",
     "fvg_trainer_last_error", "fvg_trainer_set_data", "fvg_trainer_grad", "fvg_trainer_fit", "fvg_mlp_logits",
     "fvg_checkpoint_save", "fvg_checkpoint_load", "fvg_checkpoint_write", "fvg_checkpoint_read", "fvg_monitor",
-    "fvg_conserved_totals_noalloc", "fvg_batch_create_2d",
+    "fvg_conserved_totals_noalloc", "fvg_batch_create_2d", "fvg_batch_set_teams", "fvg_batch_get_plan",
 ]
 
 MLP_NPARAM = 1601
@@ -118,6 +118,8 @@ def lib():
         L.fvg_batch_set_block.argtypes = [C.c_void_p, C.c_int]
         L.fvg_batch_get_block.argtypes = [C.c_void_p, _I]
         L.fvg_batch_tune.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_char_p, _I, _I]
+        L.fvg_batch_set_teams.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.fvg_batch_get_plan.argtypes = [C.c_void_p, _I, _I, _I]
         F = C.POINTER(C.c_float)
         L.fvg_indicator_set.argtypes = [C.c_void_p, F, C.c_int, C.c_float]
         L.fvg_indicator_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), _I, _I]
@@ -416,6 +418,17 @@ class Ensemble:
     @block.setter
     def block(self, b: int):
         self._c(lib().fvg_batch_set_block(self._h, int(b)))
+
+    def set_teams(self, max_team: int, min_chunk: int = 512, force: bool = False):
+        """Teams of workgroups for the largest 1-D instances (max_team = 1: one workgroup per instance)."""
+        self._c(lib().fvg_batch_set_teams(self._h, int(max_team), int(min_chunk), 1 if force else 0))
+
+    @property
+    def plan(self):
+        """(teams, workgroups in teams, workgroups launched) of the last run."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._c(lib().fvg_batch_get_plan(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def tune(self, t_probe: float, max_steps: int, cache_path: str | None = None):
         """Autotune the workgroup size on a probe run (the state is restored); returns (block, from_cache)."""

@@ -312,6 +312,18 @@ int fvg_batch_get_block(const fvg_batch* b, int* block);
  * architecture, instance sizes, scheme) is appended, and a later call for the same key reads it instead of
  * measuring (*from_cache = 1). */
 int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cache_path, int* block, int* from_cache);
+/* Teams of 1-D instances (no reference counterpart: the reference runs one process per instance). The largest
+ * instances run on several workgroups of one cooperative launch, each over its own chunk of at least min_chunk
+ * cells, with two barriers among a team's workgroups per stage; the other instances are taken one at a time, longest
+ * first, by any workgroup. An instance gets a team when its estimated work is at least twice the average per resident
+ * workgroup (as many workgroups as averages, at most max_team; teams use at most half of the workgroups); with
+ * force, every instance of at least 2 min_chunk cells gets min(max_team, N / min_chunk). Results do not depend on
+ * the teams. Waits at a team barrier are bounded: a team that times out makes fvg_batch_run return FVG_EHIP.
+ * max_team = 1, first-order faces or a device without cooperative launches: one workgroup per instance.
+ * Defaults: max_team 8, min_chunk 512, force 0. max_team <= 16, min_chunk >= 3. */
+int fvg_batch_set_teams(fvg_batch* b, int max_team, int min_chunk, int force);
+/* The last fvg_batch_run's launch: teams, workgroups in teams, workgroups launched. */
+int fvg_batch_get_plan(const fvg_batch* b, int* nteams, int* team_wgs, int* grid);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,86 @@ def test_full_size_ensemble_sampled_against_oracle(oracle):
     which = [int(order[k]) for k in (0, 1, 128, 300, 512, 700, 900, 1022, 1023)]
     r = check_against_oracle(oracle, ensemble_keys(), n, U, 0.15, 30, which=which)
     assert (r.steps == 30).all() and r.diverged.sum() == 0
+    # the default plan splits the largest instances into teams; the sample includes the two largest
+    with fvg.Ensemble(fvg.config_from_keys(ensemble_keys()), n) as e:
+        e.upload(U)
+        e.run(0.15, 1)
+        say("  default plan (teams, workgroups in teams, workgroups):", e.plan)
+        assert e.plan[0] > 0
+
+
+# ---- teams: the largest 1-D instances on several workgroups (fvg_batch_set_teams) ------------------------------------
+def run_teams(keys, n, U, t_end, max_steps, teams, block=None):
+    with fvg.Ensemble(fvg.config_from_keys(keys), n) as e:
+        if block:
+            e.block = block
+        e.set_teams(*teams)
+        e.upload(U)
+        r = e.run(t_end, max_steps)
+        return e.download(4), r, e.offsets, e.plan
+
+
+@pytest.mark.parametrize("name,over", SCHEMES, ids=[s[0] for s in SCHEMES])
+def test_teams_match_oracle(oracle, name, over):
+    # every instance of 6 cells or more split over up to 4 workgroups of at least 3 cells (the stencil reach): chunk
+    # ends inside the stencils of boundary faces, periodic wrap across the first and last rank, uneven chunks; the
+    # same bits as the reference's run of each instance. First-order faces keep one workgroup per instance.
+    say("ensemble teams", name)
+    keys = dict(ensemble_keys(), **over)
+    n, _, params = riemann_ensemble(10, seed=11, nmin=1, nmax=90)
+    n[:5] = [1, 2, 5, 6, 7]
+    U = riemann_states(n, params)
+    Ud, r, off, plan = run_teams(keys, n, U, 0.05, 300, (4, 3, True))
+    if keys.get("reconstruction") == "none":
+        assert plan[0] == 0
+    else:
+        assert plan[0] == int((n >= 6).sum()) and plan[1] > plan[0], plan
+    for i in range(len(n)):
+        Uo, t, steps, div = oracle_instance(oracle, keys, int(n[i]), U[off[i]:off[i + 1]], 0.05, 300)
+        assert steps == r.steps[i] and div == r.diverged[i], (i, steps, r.steps[i], div, r.diverged[i])
+        assert t == r.t[i], (i, t, r.t[i])
+        assert same(Ud[off[i]:off[i + 1]], Uo), i
+
+
+@pytest.mark.parametrize("block", [64, 256, 1024])
+def test_teams_do_not_change_bits(block):
+    # teams of up to 16 workgroups of 3 or more cells, and the automatic plan, against one workgroup per instance
+    say("ensemble teams: bits, block", block)
+    n, U, _ = riemann_ensemble(24, seed=3, nmin=16, nmax=1500)
+    for keys in (ensemble_keys(), dict(ensemble_keys(), reconstruction="weno5", time_scheme="SSP Runge-Kutta 3",
+                                        bc_xmin="periodic", bc_xmax="periodic")):
+        U1, r1, _, plan1 = run_teams(keys, n, U, 0.02, 400, (1, 512, False), block)
+        assert plan1[0] == 0
+        for teams in ((16, 3, True), (5, 40, True), (8, 64, False)):
+            Ut, rt, _, plan = run_teams(keys, n, U, 0.02, 400, teams, block)
+            say("  plan", teams, plan)
+            if teams[2]:
+                assert plan[0] > 0
+            assert same(Ut, U1) and np.array_equal(rt.steps, r1.steps) and same(rt.t, r1.t) and same(rt.s_last, r1.s_last)
+
+
+def test_teamed_divergence_is_reported(oracle):
+    # an instance that diverges inside a team: every rank leaves together and the instance reports it
+    say("ensemble teams: divergence")
+    n, U, _ = riemann_ensemble(6, seed=5, nmin=20, nmax=60)
+    off = np.concatenate([[0], np.cumsum(n)])
+    Ub = U.copy()
+    Ub[off[2] + 3] = [1.0, 0.0, 0.0, np.inf]
+    Ud, r, _, plan = run_teams(ensemble_keys(), n, Ub, 0.05, 500, (4, 5, True))
+    assert plan[0] == 6
+    assert r.diverged.tolist() == [0, 0, 1, 0, 0, 0] and r.steps[2] == 1
+    _, _, steps, div = oracle_instance(oracle, ensemble_keys(), int(n[2]), Ub[off[2]:off[3]], 0.05, 500)
+    assert div == 1 and steps == 1
+
+
+def test_team_settings_are_checked():
+    with fvg.Ensemble(fvg.config_from_keys(ensemble_keys()), [8, 9]) as e:
+        with pytest.raises(fvg.InvalidArgument):
+            e.set_teams(0)
+        with pytest.raises(fvg.InvalidArgument):
+            e.set_teams(17)
+        with pytest.raises(fvg.InvalidArgument):
+            e.set_teams(4, 2)
 
 
 def test_tuner_restores_state_and_caches(tmp_path):
