@@ -23,10 +23,11 @@ struct DevState {
     long long reverted;
     int done, diverged, steps, max_steps;
     double ut[6];   // mass-flow inlet velocity per side of the current residual (k_massflow)
-    // nonzero: a primitive the residual's k_cells stored (FP64 planes) is outside range_fits_quot6, so the face
+    // nonzero: a value k_cells stored for the residual's WENO5-Z stencils (a primitive of the FP64 planes or, with
+    // the fourth-order quadrature, a value of the padded conservative planes) is outside range_fits_quot6, so the
     // kernels take the IEEE division for WENO5-Z's quotients by 6. k_cells only sets it; k_update (the end of every
     // residual of the separate passes) and k_control (after a residual's faces or between steps) clear it, never
-    // between a k_cells and the face kernels that read its planes.
+    // between a k_cells and the kernels that read its planes.
     int wide;
 };
 

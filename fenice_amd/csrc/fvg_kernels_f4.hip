@@ -65,10 +65,9 @@ __device__ __forceinline__ void q4_load(const double* planes, long long n, long 
 }   // namespace
 
 // R: the reconstruction, 2 WENO3-Z (four cells t-2 .. t+1), 3 WENO5-Z (six cells t-3 .. t+2)
-template <int NV, int D, int R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k4_states(KP p) {
-    const long long f = xcd_block() * blockDim.x + threadIdx.x;
-    if (f >= p.nfa[D] || p.st->done) return;
+// Q6: WENO5-Z's quotients by 6 in three operations (quot6; every value of Wc in range_fits_quot6, the same bits)
+template <int NV, int D, int R, bool Q6>
+__device__ __forceinline__ void k4_states_face(const KP& p, long long f) {
     int i, j, k, t;
     face_coords<NV, D>(p, f, i, j, k, t);
     const int n = ncell(p, D);
@@ -89,8 +88,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if constexpr (R == 3) {
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
-                ul[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
-                ur[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
+                ul[q] = weno5z<double, Q6>(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
+                ur[q] = weno5z<double, Q6>(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
             }
         } else {
 #pragma unroll
@@ -147,6 +146,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         p.q4_UR[q * p.q4_n + f] = ur[q];
     }
     p.q4_good[f] = good ? 1 : 0;
+}
+
+template <int NV, int D, int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k4_states(KP p) {
+    const long long f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= p.nfa[D] || p.st->done) return;
+    if constexpr (R == 3) {   // uniform over the launch: quot6 unless a value of Wc is outside its range
+        if (__builtin_amdgcn_readfirstlane(p.st->wide) == 0) k4_states_face<NV, D, R, true>(p, f);
+        else k4_states_face<NV, D, R, false>(p, f);
+    } else {
+        k4_states_face<NV, D, R, false>(p, f);
+    }
 }
 
 template <int NV, int D>

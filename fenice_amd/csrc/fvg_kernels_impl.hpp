@@ -433,7 +433,7 @@ __device__ __forceinline__ void push_halo(const KP& p, T* Wo, int i, int j, int 
             c2p<NV>(p.rs.eos, ug, wg, (double*)nullptr);
             to_t<NV>(wg, v);
         }
-        if constexpr (!CONS && std::is_same<T, double>::value) {
+        if constexpr (std::is_same<T, double>::value) {   // FP64 primitives, or the conservative ghosts
 #pragma unroll
             for (int q = 0; q < NV; ++q) wide |= !range_fits_quot6(v[q]);
         }
@@ -490,7 +490,10 @@ __global__ __launch_bounds__(256) void k_cells(KP p) {
             c2p<NV>(p.rs.eos, u, w, (double*)nullptr);
             if (p.quad4) {   // fourth-order fluxes: the padded conservative state; the primitives only for the viscous term
 #pragma unroll
-                for (int q = 0; q < NV; ++q) p.Wc[q * p.npad + pc] = u[q];
+                for (int q = 0; q < NV; ++q) {
+                    p.Wc[q * p.npad + pc] = u[q];
+                    wide |= !range_fits_quot6(u[q]);
+                }
                 push_halo<NV, double, true>(p, p.Wc, i, j, k, u, u, wide);
             }
             if (!p.quad4 || p.viscous) {

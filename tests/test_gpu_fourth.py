@@ -106,3 +106,35 @@ def test_fourth_fp32_is_refused():
     keys = base_keys(8, 8, reconstruction="weno5", flux_quadrature="fourth", flux_precision="fp32")
     with pytest.raises(fvg.Unsupported):
         fvg.Solver(fvg.config_from_keys(keys))
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 1), (8, 9, 7)], ids=["2d", "3d"])
+def test_fourth_quotients_outside_the_fast_range(oracle, dims):
+    # subnormal transverse momenta: k_cells finds padded conservative values below 2^-900 and the conserved WENO5-Z
+    # of k4_states takes the IEEE division by 6 (Schemes.cpp:343-345); the residual and steps keep the reference's
+    # bits, and a field in range afterwards takes the three-operation quotients with the same bits
+    from test_gpu_parity import tiny_velocity_state
+    nx, ny, nz = dims
+    say(f"fourth quotients outside the fast range {dims}")
+    keys = base_keys(nx, ny, nz, reconstruction="weno5", flux_quadrature="fourth")
+    cfg = fvg.config_from_keys(keys)
+    two_d = nz == 1
+    U = tiny_velocity_state(nx, ny, nz, 7)
+    with fvg.Solver(cfg) as s:
+        s.upload(U)
+        Rd, Sd = s.residual()
+        Ro, So = oracle.residual(cfg, U)
+        assert Sd == So and same_bits(Rd, Ro, two_d)
+        Uo = U.copy()
+        t, lag = 0.0, 0.0
+        for _ in range(3):
+            r = s.step(t, 10.0)
+            dto, so, lag, ok = oracle.step(cfg, Uo, t, 10.0, 0.0, lag)
+            assert ok and r.dt == dto and r.s_max == so
+            t += r.dt
+        assert same_bits(s.download(), Uo, two_d)
+        U2 = smooth_state(nx, ny, nz, 8)
+        s.upload(U2)
+        Rd, Sd = s.residual()
+        Ro, So = oracle.residual(cfg, U2)
+        assert Sd == So and same_bits(Rd, Ro, two_d)
