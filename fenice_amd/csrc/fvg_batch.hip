@@ -178,6 +178,7 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
             for (int s = 0; s < 6; ++s) cstencil(b, in, Us, ch, f - 3 + s, c[s]);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
+                if (q == 2) continue;   // v: see below
                 wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
                 wr[q] = weno5z(c[5][q], c[4][q], c[3][q], c[2][q], c[1][q], we);
             }
@@ -188,17 +189,25 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
             if constexpr (RECON == 2) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
+                    if (q == 2) continue;
                     wl[q] = weno3z(c[0][q], c[1][q], c[2][q], we);
                     wr[q] = weno3z(c[3][q], c[2][q], c[1][q], we);
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
+                    if (q == 2) continue;
                     wl[q] = c[1][q] + 0.5 * limiter(b.lim, c[1][q] - c[0][q], c[2][q] - c[1][q]);
                     wr[q] = c[2][q] - 0.5 * limiter(b.lim, c[3][q] - c[2][q], c[2][q] - c[1][q]);
                 }
             }
         }
+        // v is +-0 in every stencil value (a no-slip ghost gives -0), so its reconstruction is a zero of some sign;
+        // v enters the Riemann solver only squared (kinetic energy, the low-Mach Mach number) or in the rho v flux,
+        // which the ensemble does not keep, so +0 gives the same (rho, rho u, E) fluxes and the reconstruction of v
+        // is skipped
+        wl[2] = 0.0;
+        wr[2] = 0.0;
         p2c<4>(b.rs.eos, wl, UL);
         p2c<4>(b.rs.eos, wr, UR);
     } else {   // boundary face without a halo: first-order states through the primitive round trip
@@ -216,12 +225,15 @@ __device__ __forceinline__ void bface(const BP& b, const BInst& in, const double
 // without its flux. Both states are u and the normal velocity is +0 (rho v = +0), so the Einfeldt bounds are
 // S_L = 0 - c = -c and S_R = c, with c the sound speed of c2p(u), and max(|S_L|, |S_R|) = c; the low-Mach variant
 // takes c before its velocity blend, which leaves equal states unchanged.
+// w and inv: c2p of the cell's state, as the primitive pass computes them (the round trip's first half)
 template <int RECON>
-__device__ __forceinline__ double by_speed(const BP& b, const BInst& in, const double* Us, int i) {
-    double u[4], w[4], inv;
-    bload_cons(b, in, Us, i, u);
-    if constexpr (RECON != 0) roundtrip<4>(b.rs.eos, u);
-    c2p<4>(b.rs.eos, u, w, &inv);
+__device__ __forceinline__ double by_speed(const BP& b, const double* w, double inv) {
+    if constexpr (RECON != 0) {   // the face state is the round trip p2c(c2p(u))
+        double u2[4], w2[4], inv2;
+        p2c<4>(b.rs.eos, w, u2);
+        c2p<4>(b.rs.eos, u2, w2, &inv2);
+        return sound_speed_inv(b.rs.eos, w2[3], inv2);
+    }
     return sound_speed_inv(b.rs.eos, w[3], inv);
 }
 
@@ -311,14 +323,14 @@ __device__ void run_instance(const BP& b, int id, int r, int m, TeamSync* ts, do
             // primitives of the stage input (k_cells) and the wave speed of each cell's y faces
             double local = 0.0;
             for (int i = ch.lo + (int)threadIdx.x; i < ch.hi; i += BS) {
-                double u[4], w[4];
+                double u[4], w[4], inv;
                 bload_cons(b, in, Us, i, u);
-                c2p<4>(b.rs.eos, u, w, (double*)nullptr);
+                c2p<4>(b.rs.eos, u, w, &inv);
                 const long long c = in.c0 + i;
                 b.W[c] = w[0];
                 b.W[b.nc + c] = w[1];
                 b.W[2 * b.nc + c] = w[3];
-                const double sy = by_speed<RECON>(b, in, Us, i);
+                const double sy = by_speed<RECON>(b, w, inv);
                 if (sy > local) local = sy;
             }
             if (team) {
