@@ -460,9 +460,10 @@ __device__ void run_instance(const BP& b, int id, int r, int m, TeamSync* ts, do
 
 // Workgroups [0, team_wgs) run the teams' instances, then every workgroup takes the queued instances one at a time
 // (longest first) until none is left. Without a queue, workgroup k runs instance order[k] alone.
-// 512 threads: at most 168 VGPRs (three wavefronts per SIMD), measured faster than the compiler's 210 (14.4 against
-// 15.7 s on the 1024-instance run); the other sizes keep their default limits
-template <int BS> constexpr int kEnsembleWaves = BS == 512 ? 3 : 1;
+// 512 and 768 threads: at most 168 VGPRs (three wavefronts per SIMD; with 768 threads a CU runs 12 wavefronts of one
+// workgroup), measured faster than the compiler's 210 at 512 threads (14.4 against 15.7 s on the 1024-instance run);
+// the other sizes keep their default limits
+template <int BS> constexpr int kEnsembleWaves = (BS == 512 || BS == 768) ? 3 : 1;
 template <int BS, int RECON>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kEnsembleWaves<BS>))) void k_ensemble(BP b) {
     __shared__ double red[BS / 64];
@@ -716,7 +717,7 @@ using namespace fvg;
 
 namespace {
 std::string g_batch_create_error;
-const int kBlocks[] = {64, 128, 256, 512, 1024};
+const int kBlocks[] = {64, 128, 256, 512, 768, 1024};
 }   // namespace
 
 struct fvg_batch {
@@ -936,6 +937,7 @@ int launch_ensemble(fvg_batch* b, const BP& p, int block) {
         case 128: return launch_teams_bs<128>(b, p);
         case 256: return launch_teams_bs<256>(b, p);
         case 512: return launch_teams_bs<512>(b, p);
+        case 768: return launch_teams_bs<768>(b, p);
         default: return launch_teams_bs<1024>(b, p);
         }
     }
@@ -947,6 +949,7 @@ int launch_ensemble(fvg_batch* b, const BP& p, int block) {
     case 128: launch_bs<128>(b, p); break;
     case 256: launch_bs<256>(b, p); break;
     case 512: launch_bs<512>(b, p); break;
+    case 768: launch_bs<768>(b, p); break;
     default: launch_bs<1024>(b, p); break;
     }
     BCHK(b, hipGetLastError());
@@ -1308,7 +1311,7 @@ int fvg_batch_set_block(fvg_batch* b, int block) {
             b->block = block;
             return FVG_OK;
         }
-    return bset(b, FVG_EINVAL, "block must be 64, 128, 256, 512 or 1024");
+    return bset(b, FVG_EINVAL, "block must be 64, 128, 256, 512, 768 or 1024");
 }
 
 int fvg_batch_set_teams(fvg_batch* b, int max_team, int min_chunk, int force) {

@@ -304,7 +304,7 @@ int fvg_batch_download(fvg_batch* b, double* U, long long ncells, int ncomp);
  * output pointer may be NULL. Divergence of an instance is reported through its flag, not the status. */
 int fvg_batch_run(fvg_batch* b, double t_end, double dt_max, int max_steps, double* t, int* steps, int* diverged, double* s_last,
                   double* ms);
-/* Workgroup size of the ensemble kernel: 64, 128, 256 (default), 512 or 1024. Results do not depend on it. */
+/* Workgroup size of the ensemble kernel: 64, 128, 256 (default), 512, 768 or 1024. Results do not depend on it. */
 int fvg_batch_set_block(fvg_batch* b, int block);
 int fvg_batch_get_block(const fvg_batch* b, int* block);
 /* Autotuner: time every workgroup size on a probe run of the uploaded state (to t_probe, at most max_steps),

@@ -75,7 +75,7 @@ def test_scheme_matrix_matches_oracle(oracle, name, over):
 def test_block_size_does_not_change_bits():
     say("ensemble: block sizes")
     n, U, _ = riemann_ensemble(24, seed=3, nmin=16, nmax=1500)
-    outs = [run_device(ensemble_keys(), n, U, 0.02, 400, b) for b in (64, 256, 1024)]
+    outs = [run_device(ensemble_keys(), n, U, 0.02, 400, b) for b in (64, 256, 768, 1024)]
     for Ud, r, _ in outs[1:]:
         assert same(Ud, outs[0][0]) and np.array_equal(r.steps, outs[0][1].steps) and same(r.t, outs[0][1].t)
 
@@ -144,7 +144,7 @@ def test_teams_match_oracle(oracle, name, over):
         assert same(Ud[off[i]:off[i + 1]], Uo), i
 
 
-@pytest.mark.parametrize("block", [64, 256, 1024])
+@pytest.mark.parametrize("block", [64, 256, 768, 1024])
 def test_teams_do_not_change_bits(block):
     # teams of up to 16 workgroups of 3 or more cells, and the automatic plan, against one workgroup per instance
     say("ensemble teams: bits, block", block)
@@ -193,7 +193,7 @@ def test_tuner_restores_state_and_caches(tmp_path):
     with fvg.Ensemble(fvg.config_from_keys(keys), n) as e:
         e.upload(U)
         b1, fc1 = e.tune(0.005, 200, cache)
-        assert not fc1 and b1 in (64, 128, 256, 512, 1024) and e.block == b1
+        assert not fc1 and b1 in (64, 128, 256, 512, 768, 1024) and e.block == b1
         assert same(e.download(4), U)
         r = e.run(0.02, 400)
         Ut = e.download(4)
@@ -280,7 +280,7 @@ def test_2d_block_sizes_and_tuner(oracle):
     nx, ny = np.array([s[0] for s in sizes]), np.array([s[1] for s in sizes])
     U = np.concatenate([vortex_rect(a, b) for a, b in sizes])
     outs = []
-    for block in (64, 256, 1024):
+    for block in (64, 256, 768, 1024):
         with fvg.Ensemble(fvg.config_from_keys(keys), nx, ny=ny) as e:
             e.block = block
             e.upload(U)
