@@ -175,6 +175,27 @@ def test_teamed_divergence_is_reported(oracle):
     assert div == 1 and steps == 1
 
 
+@pytest.mark.parametrize("over", [dict(cfl="0.9"), dict(reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.9")],
+                         ids=["muscl-rk2", "weno5-rk3"])
+def test_teamed_divergence_matches_single_workgroups(over):
+    # instances that diverge inside teams, or see extreme values: time, step count, last wave speed, flags and state
+    # as with one workgroup per instance
+    say("ensemble teams: divergence against single workgroups,", over)
+    keys = dict(ensemble_keys(), **over)
+    n, U, _ = riemann_ensemble(8, seed=21, nmin=40, nmax=120)
+    off = np.concatenate([[0], np.cumsum(n)])
+    Ub = U.copy()
+    Ub[off[1] + 5] = [1.0, 0.0, 0.0, np.inf]      # the first step's maximum is infinite
+    Ub[off[4] + 7] = [1.0, 0.0, 0.0, np.nan]      # a NaN energy (the primitive floors may absorb it)
+    Ub[off[6] + 2, 3] = 1e300                     # an extreme energy
+    U1, r1, _, plan1 = run_teams(keys, n, Ub, 0.05, 500, (1, 512, False))
+    Ut, rt, _, plan = run_teams(keys, n, Ub, 0.05, 500, (4, 10, True))
+    assert plan1[0] == 0 and plan[0] == 8
+    assert r1.diverged[1] == 1 and r1.diverged.sum() < 8
+    assert np.array_equal(r1.diverged, rt.diverged) and np.array_equal(r1.steps, rt.steps)
+    assert same(r1.t, rt.t) and same(r1.s_last, rt.s_last) and same(U1, Ut)
+
+
 def test_team_settings_are_checked():
     with fvg.Ensemble(fvg.config_from_keys(ensemble_keys()), [8, 9]) as e:
         with pytest.raises(fvg.InvalidArgument):
