@@ -354,6 +354,7 @@ def side_workload(name: str) -> dict:
             e.upload(Uv)
             vblock, _ = e.tune(0.05, 100000)
             rv = e.run(1.0, 100000)
+            vplan = e.plan
         vstages = 2.0 * float(((nv.astype(np.float64) ** 2) * rv.steps).sum())
         return {"workload": "1024 1-D Riemann problems, 128..8192 cells (4,243,015), MUSCL MC + HLLC + SSP-RK2, CFL 0.5, "
                             "to t = 0.15", "block": block, "kernel_ms": round(r.ms, 2),
@@ -366,6 +367,7 @@ def side_workload(name: str) -> dict:
                                                  f"({int((nv.astype(np.int64) ** 2).sum()):,} cells), MUSCL Van Leer + HLLC + "
                                                  "SSP-RK2, CFL 0.5, to t = 1",
                                      "block": vblock, "kernel_ms": round(rv.ms, 2),
+                                     "teams": {"teams": vplan[0], "workgroups_in_teams": vplan[1], "workgroups": vplan[2]},
                                      "value": round(vstages / (rv.ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s per RK stage",
                                      "instances_done": int((rv.t >= 1.0).sum()), "diverged": int(rv.diverged.sum()),
                                      "steps_max": int(rv.steps.max()), "steps_total": int(rv.steps.sum())}}

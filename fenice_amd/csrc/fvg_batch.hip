@@ -47,6 +47,7 @@ struct BOut {
 constexpr int KH = 3;   // stencil reach beyond a chunk: the WENO5-Z faces of a chunk read 3 cells on each side
 struct BTeam {
     int inst, wg0, m, pad;   // instance, first workgroup, workgroups
+    long long xoff;          // 2-D instances: the team's exchange rows in BP2::xch, from this element
 };
 struct TeamSync {
     unsigned long long count;    // barrier arrivals so far; barrier k of the run completes at (k + 1) m
@@ -526,6 +527,16 @@ struct BP2 {
     const BInst2* inst;
     const int* order;
     BOut* out;
+    // teams and queue as for 1-D instances (BP); a team's ranks take row slabs of its instance, and exchange the
+    // primitives of their KH end rows through xch (per team from BTeam::xoff: [rank][side][q][KH][nx] doubles, then
+    // [rank][q][nx], the y-face fluxes of a rank's last face row, which the next rank owns)
+    const BTeam* teams;
+    int nteams, team_wgs;
+    const int* queue;
+    int nqueue, pad;
+    unsigned* qnext;
+    TeamSync* sync;
+    double* xch;
 };
 
 __device__ __forceinline__ void b2load(const double* P, long long stride, long long c, double* v) {
@@ -556,6 +567,35 @@ __device__ __forceinline__ void b2stencil(const BP2& b, const BInst2& in, const 
     }
 }
 
+// The rows [lo, hi) of a 2-D instance that a team's rank owns, and the exchange rows of its team: the primitives of
+// the KH rows below lo (has_lo) and from hi (has_hi) are the neighbours' end rows there (across the ends of a
+// periodic y the rows of the other end).
+struct Chunk2 {
+    int lo, hi;
+    bool has_lo, has_hi;
+    const double* xl;   // low neighbour's high end rows: [q][KH][nx]
+    const double* xh;   // high neighbour's low end rows
+};
+
+// the padded primitive state at row t of column i (y stencils) as the chunk's faces read it
+__device__ __forceinline__ void c2stencil(const BP2& b, const BInst2& in, const double* Us, const Chunk2& ch, int i, int t, double* w) {
+    const double* x = nullptr;
+    int k = 0;
+    if (t < ch.lo && ch.has_lo) {
+        x = ch.xl;
+        k = t - ch.lo + KH;
+    } else if (t >= ch.hi && ch.has_hi) {
+        x = ch.xh;
+        k = t - ch.hi;
+    }
+    if (x) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = team_get(x + ((long long)q * KH + k) * in.nx + i);
+        return;
+    }
+    b2stencil(b, in, Us, 1, i, 0, t, w);
+}
+
 // the first-order state beyond side 2d + hi of the boundary cell of the line through (i, j) (outside_first_order)
 __device__ __forceinline__ void b2outside(const BP2& b, const BInst2& in, const double* Us, int d, int hi, int i, int j, double* ug) {
     const int n = d == 0 ? in.nx : in.ny, side = 2 * d + hi;
@@ -572,7 +612,8 @@ __device__ __forceinline__ void b2outside(const BP2& b, const BInst2& in, const 
 // face t along axis d of the line through (i, j): the states of face_states (MODE_PLAIN / MODE_FIRST_ORDER) and the
 // Riemann flux
 template <int RECON>
-__device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const double* Us, int d, int i, int j, int t, double* F, double& sm) {
+__device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const double* Us, const Chunk2& ch, int d, int i, int j, int t, double* F,
+                                       double& sm) {
     const int n = d == 0 ? in.nx : in.ny;
     const bool lo = t == 0, hi = t == n;
     auto cell = [&](int tt) { return in.c0 + (long long)(d == 1 ? tt : j) * in.nx + (d == 0 ? tt : i); };
@@ -584,7 +625,7 @@ __device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const dou
         if constexpr (RECON == 3) {
             double c[6][4];
 #pragma unroll
-            for (int s = 0; s < 6; ++s) b2stencil(b, in, Us, d, i, j, t - 3 + s, c[s]);
+            for (int s = 0; s < 6; ++s) d ? c2stencil(b, in, Us, ch, i, t - 3 + s, c[s]) : b2stencil(b, in, Us, 0, i, j, t - 3 + s, c[s]);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 wl[q] = weno5z(c[0][q], c[1][q], c[2][q], c[3][q], c[4][q], we);
@@ -593,7 +634,7 @@ __device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const dou
         } else {
             double c[4][4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) b2stencil(b, in, Us, d, i, j, t - 2 + s, c[s]);
+            for (int s = 0; s < 4; ++s) d ? c2stencil(b, in, Us, ch, i, t - 2 + s, c[s]) : b2stencil(b, in, Us, 0, i, j, t - 2 + s, c[s]);
             if constexpr (RECON == 2) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -623,25 +664,57 @@ __device__ __forceinline__ void b2face(const BP2& b, const BInst2& in, const dou
     face_flux<4>(b.rs, UL, UR, d, F, sm);
 }
 
+// The run of 2-D instance id by rank r of a team of m workgroups (m = 1: the workgroup alone), over the rows
+// [lo, hi) of its slab, with the barriers and the exchange of run_instance: after the primitives the ranks publish
+// their KH end rows (every wavefront stores, then waits for its stores, before the workgroup barrier that precedes
+// the arrival) and the y stencils read the neighbours' from xch; after the faces the team takes the maximum of the
+// wave speeds. The y faces of a slab are rows lo .. hi; row hi, which the next rank owns, is computed again for the
+// slab's last row and kept in the rank's own part of xch.
 template <int BS, int RECON>
-__global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
-    __shared__ double red[BS / 64];
-    const int id = b.order[blockIdx.x];
+__device__ void run_instance2(const BP2& b, int id, int r, int m, TeamSync* ts, double* xch, double* red, int* flag) {
     const BInst2 in = b.inst[id];
-    const int nx = in.nx, ny = in.ny, n = nx * ny;
-    const int nfx = (nx + 1) * ny, nfy = nx * (ny + 1);
+    const int nx = in.nx, ny = in.ny;
     const int nst = b.scheme + 1;
+    const bool team = m > 1;
+    const bool per = in.halo[2] == 1;   // periodic y: the first rank's low neighbour is the last rank
+    const long long rows_x = 2LL * 4 * KH * nx;   // exchange rows of one rank
+    Chunk2 ch;
+    ch.lo = (int)((long long)ny * r / m);
+    ch.hi = (int)((long long)ny * (r + 1) / m);
+    ch.has_lo = team && (r > 0 || per);
+    ch.has_hi = team && (r < m - 1 || per);
+    const int r_lo = (r + m - 1) % m, r_hi = (r + 1) % m;
+    ch.xl = team ? xch + r_lo * rows_x + 4LL * KH * nx : nullptr;   // the low neighbour's high end
+    ch.xh = team ? xch + r_hi * rows_x : nullptr;                    // the high neighbour's low end
+    double* ytop = team ? xch + m * rows_x + (long long)r * 4 * nx : nullptr;   // this rank's copy of face row hi
+    const bool extra = team && r < m - 1;
+    const int c0 = ch.lo * nx, c1 = ch.hi * nx;   // the slab's cells
+    const int nfx = (nx + 1) * (ch.hi - ch.lo), nfy = nx * (ch.hi - ch.lo + 1);
+    unsigned long long nbar = 0;
     double t = 0.0, dt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s_step = 0.0;
     int steps = 0, diverged = 0;
     while (t < b.t_end && steps < b.max_steps) {
         for (int st = 1; st <= nst; ++st) {
             const double* Us = st == 1 ? b.U : b.Ut;
-            for (int c = threadIdx.x; c < n; c += BS) {   // primitives of the stage input (k_cells)
+            for (int c = c0 + (int)threadIdx.x; c < c1; c += BS) {   // primitives of the stage input (k_cells)
                 double u[4], w[4];
                 b2load(Us, b.nc, in.c0 + c, u);
                 c2p<4>(b.rs.eos, u, w, (double*)nullptr);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) b.W[q * b.nc + in.c0 + c] = w[q];
+            }
+            if (team) {   // the slab's KH end rows, side 0 the low end and side 1 the high end
+                __syncthreads();
+                for (int e = threadIdx.x; e < 2 * 4 * KH * nx; e += BS) {
+                    const int side = e / (4 * KH * nx), q = (e / (KH * nx)) % 4, k = (e / nx) % KH, i = e % nx;
+                    const int row = side ? ch.hi - KH + k : ch.lo + k;
+                    team_put(xch + r * rows_x + e, b.W[q * b.nc + in.c0 + (long long)row * nx + i]);
+                }
+                __builtin_amdgcn_s_waitcnt(0);   // every wavefront's published rows are complete
+                if (!team_barrier(ts, m, nbar, flag)) {
+                    diverged = 2;
+                    break;
+                }
             }
             __syncthreads();
             double local = 0.0;
@@ -649,15 +722,40 @@ __global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
                 const int d = f < nfx ? 0 : 1;
                 const int g = d == 0 ? f : f - nfx;
                 const int w = d == 0 ? nx + 1 : nx;
-                const int j = g / w, i = g - j * w;   // x faces: face i of row j; y faces: face j of column i
+                const int jj = g / w, ii = g - jj * w;   // x faces: face ii of slab row jj; y faces: face row jj of column ii
+                const int j = ch.lo + jj;
                 double F[4], sm;
-                b2face<RECON>(b, in, Us, d, d == 0 ? 0 : i, d == 0 ? j : 0, d == 0 ? i : j, F, sm);
+                b2face<RECON>(b, in, Us, ch, d, d == 0 ? 0 : ii, d == 0 ? j : 0, d == 0 ? ii : j, F, sm);
                 if (sm > local) local = sm;
-                const long long x = (d == 0 ? in.fx0 : in.fy0) + g;
+                if (d == 1 && extra && j == ch.hi) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) b.F[q * b.nf + x] = F[q];
+                    for (int q = 0; q < 4; ++q) ytop[q * nx + ii] = F[q];
+                } else {
+                    const long long x = d == 0 ? in.fx0 + (long long)j * (nx + 1) + ii : in.fy0 + (long long)j * nx + ii;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) b.F[q * b.nf + x] = F[q];
+                }
             }
-            const double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            double S = bmax_all<BS>(local, red);   // its barriers also publish the face fluxes
+            if (team) {   // the team's maximum
+                if (threadIdx.x == 0) team_put(&ts->smax[r], S);
+                if (!team_barrier(ts, m, nbar, flag)) {
+                    diverged = 2;
+                    break;
+                }
+                if (threadIdx.x < 64) {   // as in run_instance: a NaN of any rank is kept
+                    double v = threadIdx.x < (unsigned)m ? team_get(&ts->smax[threadIdx.x]) : 0.0;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const double w = __shfl_xor(v, o, 64);
+                        v = (v < w || w != w) ? w : v;
+                    }
+                    if (threadIdx.x == 0) red[0] = v;
+                }
+                __syncthreads();
+                S = red[0];
+                __syncthreads();
+            }
             if (st == 1) {   // k_control
                 s1 = S;
                 dt = smin(b.cfl * in.Ldt / (S + kDtEps), b.t_end - t);
@@ -667,13 +765,14 @@ __global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
             } else {
                 s3 = S;
             }
-            for (int c = threadIdx.x; c < n; c += BS) {   // balance and stage combination (k_update)
+            for (int c = c0 + (int)threadIdx.x; c < c1; c += BS) {   // balance and stage combination (k_update)
                 const int j = c / nx, i = c - j * nx;
                 const long long fx = in.fx0 + (long long)j * (nx + 1) + i, fy = in.fy0 + (long long)j * nx + i;
+                const bool top = extra && j == ch.hi - 1;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const double* Fq = b.F + q * b.nf;
-                    const double fl = Fq[fx], fr = Fq[fx + 1], fb = Fq[fy], ft = Fq[fy + nx];
+                    const double fl = Fq[fx], fr = Fq[fx + 1], fb = Fq[fy], ft = top ? ytop[q * nx + i] : Fq[fy + nx];
                     const double R = ((0.0 - (fr * in.Ax - fl * in.Ax)) - (ft * in.Ay - fb * in.Ay)) * in.inv_vol;
                     const long long y = q * b.nc + in.c0 + c;
                     if (b.scheme == 0) {
@@ -691,6 +790,7 @@ __global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
             }
             __syncthreads();
         }
+        if (diverged == 2) break;
         s_step = b.scheme == 0 ? s1 : (b.scheme == 1 ? smax(s1, s2) : smax(s1, smax(s2, s3)));
         steps += 1;
         if (!(isfinite(s_step) && isfinite(dt) && t + dt > t)) {   // check_progress (SimulationInternal.hpp:259-267)
@@ -699,7 +799,7 @@ __global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
         }
         t += dt;
     }
-    if (threadIdx.x == 0) {
+    if (r == 0 && threadIdx.x == 0) {
         BOut o;
         o.t = t;
         o.s_last = s_step;
@@ -707,6 +807,43 @@ __global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
         o.steps = steps;
         o.diverged = diverged;
         b.out[id] = o;
+    }
+    __syncthreads();
+}
+
+// as k_ensemble: workgroups [0, team_wgs) run the teams' instances, then every workgroup takes queued instances;
+// without a queue, workgroup k runs instance order[k] alone
+template <int BS, int RECON>
+__global__ __launch_bounds__(BS) void k_ensemble2(BP2 b) {
+    __shared__ double red[BS / 64];
+    __shared__ int flag, slot;
+    const int wg = blockIdx.x;
+    int phase = !b.queue ? 0 : (wg < b.team_wgs ? 1 : 2);
+    for (;;) {
+        int id, r = 0, m = 1;
+        TeamSync* ts = nullptr;
+        double* xch = nullptr;
+        if (phase == 0) {
+            id = b.order[wg];
+        } else if (phase == 1) {
+            int k = 0;
+            while (wg >= b.teams[k].wg0 + b.teams[k].m) ++k;
+            id = b.teams[k].inst;
+            r = wg - b.teams[k].wg0;
+            m = b.teams[k].m;
+            ts = b.sync + k;
+            xch = b.xch + b.teams[k].xoff;
+        } else {
+            if (threadIdx.x == 0) slot = (int)atomicAdd(b.qnext, 1u);
+            __syncthreads();
+            const int k = slot;
+            __syncthreads();
+            if (k >= b.nqueue) break;
+            id = b.queue[k];
+        }
+        run_instance2<BS, RECON>(b, id, r, m, ts, xch, red, &flag);
+        if (phase == 0) break;
+        phase = 2;
     }
 }
 
@@ -751,6 +888,8 @@ struct fvg_batch {
     int* d_queue = nullptr;
     unsigned* d_qnext = nullptr;
     TeamSync* d_sync = nullptr;
+    double* d_xch = nullptr;             // 2-D teams: the exchange rows (BP2::xch), grown as plans need
+    long long xch_cap = 0;               // its size in doubles
     int plan_teams = 0, plan_team_wgs = 0, plan_grid = 0;   // the last launch's plan
 };
 
@@ -783,6 +922,7 @@ void free_batch(fvg_batch* b) {
     hipFree(b->d_queue);
     hipFree(b->d_qnext);
     hipFree(b->d_sync);
+    hipFree(b->d_xch);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -828,7 +968,13 @@ void plan_teams(const fvg_batch* b, int grid, int bs, std::vector<BTeam>& teams,
     auto cap = [&](int id) {   // most workgroups instance id may take
         const int n = b->n[id];
         if (b->max_team < 2 || b->min_chunk < KH || n < 2 * b->min_chunk) return 1;
-        return std::min(b->max_team, std::min(kMaxTeam, n / b->min_chunk));
+        int k = std::min(b->max_team, std::min(kMaxTeam, n / b->min_chunk));
+        if (b->dim == 2) {   // slabs of whole rows, at least KH of them and min_chunk cells
+            const int nx = b->inst2[id].nx, ny = b->inst2[id].ny;
+            const int rows = std::max(KH, (b->min_chunk + nx - 1) / nx);
+            k = std::min(k, ny / rows);
+        }
+        return std::max(1, k);
     };
     int wgs = 0;
     if (b->force_teams) {
@@ -925,12 +1071,84 @@ int launch_teams_bs(fvg_batch* b, const BP& p) {
     }
 }
 
-// whether a launch of the 1-D kernel takes teams: cooperative launches available, a reconstruction (first-order
-// faces read the conservative state of the next chunk, plain memory of another member, which the fence-free team
-// barrier does not order) and max_team > 1
-bool teamed(const fvg_batch* b, const BP& p) { return b->dim == 1 && b->coop && b->max_team > 1 && p.recon != 0; }
+// The teamed launch of the 2-D kernel: as launch_teams, with each team's exchange rows in d_xch.
+template <int BS, int RECON>
+int launch_teams2(fvg_batch* b, const BP& p) {
+    int per_cu = 0, dev_cus = 0;
+    BCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ensemble2<BS, RECON>, BS, 0));
+    BCHK(b, hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const int resident = per_cu * dev_cus;
+    if (resident < 1) return bset(b, FVG_EHIP, "the ensemble kernel cannot be resident");
+    std::vector<BTeam> teams;
+    std::vector<int> queue;
+    plan_teams(b, resident, BS, teams, queue);
+    int wgs = 0;
+    long long xn = 0;
+    for (BTeam& t : teams) {
+        wgs += t.m;
+        t.xoff = xn;
+        xn += (long long)t.m * (2 * 4 * KH + 4) * b->inst2[t.inst].nx;
+    }
+    if (xn > b->xch_cap) {
+        BCHK(b, hipStreamSynchronize(b->stream));
+        if (b->d_xch) BCHK(b, hipFree(b->d_xch));
+        b->d_xch = nullptr;
+        b->xch_cap = 0;
+        BCHK(b, hipMalloc(&b->d_xch, sizeof(double) * (size_t)xn));
+        b->xch_cap = xn;
+    }
+    const int grid = std::min(resident, wgs + (int)queue.size());
+    if (!teams.empty()) BCHK(b, hipMemcpy(b->d_teams, teams.data(), sizeof(BTeam) * teams.size(), hipMemcpyHostToDevice));
+    if (!queue.empty()) BCHK(b, hipMemcpy(b->d_queue, queue.data(), sizeof(int) * queue.size(), hipMemcpyHostToDevice));
+    BCHK(b, hipMemsetAsync(b->d_qnext, 0, sizeof(unsigned), b->stream));
+    if (!teams.empty()) BCHK(b, hipMemsetAsync(b->d_sync, 0, sizeof(TeamSync) * teams.size(), b->stream));
+    BP2 q = b->base2;
+    q.t_end = p.t_end;
+    q.dt_max = p.dt_max;
+    q.max_steps = p.max_steps;
+    q.teams = b->d_teams;
+    q.nteams = (int)teams.size();
+    q.team_wgs = wgs;
+    q.queue = b->d_queue;
+    q.nqueue = (int)queue.size();
+    q.qnext = b->d_qnext;
+    q.sync = b->d_sync;
+    q.xch = b->d_xch;
+    void* args[] = {&q};
+    BCHK(b, hipLaunchCooperativeKernel((const void*)k_ensemble2<BS, RECON>, dim3(grid), dim3(BS), args, 0, b->stream));
+    b->plan_teams = (int)teams.size();
+    b->plan_team_wgs = wgs;
+    b->plan_grid = grid;
+    return FVG_OK;
+}
+
+template <int BS>
+int launch_teams2_bs(fvg_batch* b, const BP& p) {
+    switch (b->base2.recon) {
+    case 1: return launch_teams2<BS, 1>(b, p);
+    case 2: return launch_teams2<BS, 2>(b, p);
+    default: return launch_teams2<BS, 3>(b, p);
+    }
+}
+
+// whether a launch takes teams: cooperative launches available, a reconstruction (first-order faces read the
+// conservative state of the next chunk, plain memory of another member, which the fence-free team barrier does not
+// order) and max_team > 1
+bool teamed(const fvg_batch* b, const BP& p) {
+    return b->coop && b->max_team > 1 && (b->dim == 2 ? b->base2.recon : p.recon) != 0;
+}
 
 int launch_ensemble(fvg_batch* b, const BP& p, int block) {
+    if (teamed(b, p) && b->dim == 2) {
+        switch (block) {
+        case 64: return launch_teams2_bs<64>(b, p);
+        case 128: return launch_teams2_bs<128>(b, p);
+        case 256: return launch_teams2_bs<256>(b, p);
+        case 512: return launch_teams2_bs<512>(b, p);
+        case 768: return launch_teams2_bs<768>(b, p);
+        default: return launch_teams2_bs<1024>(b, p);
+        }
+    }
     if (teamed(b, p)) {
         switch (block) {
         case 64: return launch_teams_bs<64>(b, p);
@@ -1183,8 +1401,16 @@ int fvg_batch_create_2d(const fvg_config* cfg_in, int ninst, const int* nx, cons
     if (hipMalloc(&b->U, cb) != hipSuccess || hipMalloc(&b->Ut, cb) != hipSuccess || hipMalloc(&b->W, cb) != hipSuccess ||
         hipMalloc(&b->Usave, cb) != hipSuccess || hipMalloc(&b->F, fb) != hipSuccess ||
         hipMalloc(&b->d_inst2, sizeof(BInst2) * ninst) != hipSuccess || hipMalloc(&b->d_order, sizeof(int) * ninst) != hipSuccess ||
-        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess)
+        hipMalloc(&b->d_out, sizeof(BOut) * ninst) != hipSuccess || hipMalloc(&b->d_teams, sizeof(BTeam) * ninst) != hipSuccess ||
+        hipMalloc(&b->d_queue, sizeof(int) * ninst) != hipSuccess || hipMalloc(&b->d_qnext, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&b->d_sync, sizeof(TeamSync) * ninst) != hipSuccess)
         return fail(FVG_ENOMEM, "hipMalloc of the ensemble buffers");
+    int coop = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) == hipSuccess) b->coop = coop != 0;
+    // work per instance, first estimated as cells x the larger side (the steps grow with the resolution); longest first
+    b->cost.resize(ninst);
+    for (int i = 0; i < ninst; ++i) b->cost[i] = (double)b->n[i] * std::max(nx[i], ny[i]);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return b->cost[a] > b->cost[c]; });
     if (hipMemcpy(b->d_inst2, b->inst2.data(), sizeof(BInst2) * ninst, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_order, b->order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice) != hipSuccess)
         return fail(FVG_EHIP, "hipMemcpy of the instance table");

@@ -294,6 +294,59 @@ def test_2d_instances_match_oracle(oracle, name, over):
         assert same(Ud[off[i]:off[i + 1]], Uo), (i, a, b)
 
 
+@pytest.mark.parametrize("name,over", SCHEMES_2D, ids=[s[0] for s in SCHEMES_2D])
+def test_2d_teams_match_oracle(oracle, name, over):
+    # every instance of 6 cells and 3 rows or more split into slabs of at least 3 rows over up to 4 workgroups
+    # (periodic wrap across the first and last rank, walls, inlets and ghost slabs at the ends, a one-column mesh,
+    # uneven slabs): state, time and step count as the oracle's 2-D run of each mesh. First-order faces keep one
+    # workgroup per instance.
+    say("ensemble 2-D teams", name)
+    keys = vortex_ensemble_keys(**over)
+    nx, ny = np.array([s[0] for s in SIZES_2D]), np.array([s[1] for s in SIZES_2D])
+    U = np.concatenate([vortex_rect(a, b) for a, b in SIZES_2D])
+    t_end, max_steps = 1.0, 40
+    with fvg.Ensemble(fvg.config_from_keys(keys), nx, ny=ny) as e:
+        e.set_teams(4, 3, True)
+        e.upload(U)
+        r = e.run(t_end, max_steps)
+        Ud = e.download(4)
+        off = e.offsets
+        plan = e.plan
+    say("  plan", plan)
+    if keys.get("reconstruction") == "none":
+        assert plan[0] == 0
+    else:
+        assert plan[0] == 6 and plan[1] == 19, plan
+    for i, (a, b) in enumerate(SIZES_2D):
+        Uo, t, steps, div = oracle_instance_2d(oracle, keys, a, b, U[off[i]:off[i + 1]], t_end, max_steps)
+        assert steps == r.steps[i] and div == r.diverged[i], (i, steps, r.steps[i], div, r.diverged[i])
+        assert t == r.t[i], (i, t, r.t[i])
+        assert same(Ud[off[i]:off[i + 1]], Uo), (i, a, b)
+
+
+@pytest.mark.parametrize("block", [256, 768, 1024])
+def test_2d_teams_do_not_change_bits(block):
+    # larger vortices on teams of up to 16 workgroups and on the automatic plan, against one workgroup per instance
+    say("ensemble 2-D teams: bits, block", block)
+    keys = vortex_ensemble_keys(reconstruction="weno5", time_scheme="SSP Runge-Kutta 3", cfl="0.5")
+    sizes = [(96, 80), (64, 64), (40, 72), (24, 24)]
+    nx, ny = np.array([s[0] for s in sizes]), np.array([s[1] for s in sizes])
+    U = np.concatenate([vortex_rect(a, b) for a, b in sizes])
+    outs = []
+    for teams in ((1, 512, False), (16, 3, True), (8, 256, False)):
+        with fvg.Ensemble(fvg.config_from_keys(keys), nx, ny=ny) as e:
+            e.block = block
+            e.set_teams(*teams)
+            e.upload(U)
+            r = e.run(0.5, 60)
+            say("  plan", teams, e.plan)
+            if teams[2]:
+                assert e.plan[0] == 4
+            outs.append((e.download(4), r))
+    for Ud, r in outs[1:]:
+        assert same(Ud, outs[0][0]) and np.array_equal(r.steps, outs[0][1].steps) and same(r.t, outs[0][1].t)
+
+
 def test_2d_block_sizes_and_tuner(oracle):
     # the workgroup size leaves the bits unchanged; the tuner restores the uploaded state
     keys = vortex_ensemble_keys()
