@@ -317,8 +317,8 @@ int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cach
  * cells, with two barriers among a team's workgroups per stage; the other instances are taken one at a time, longest
  * first, by any workgroup. An instance gets a team when its estimated work is at least twice the average per resident
  * workgroup (as many workgroups as averages, at most max_team; teams use at most half of the workgroups); with
- * force, every instance of at least 2 min_chunk cells gets min(max_team, N / min_chunk). Results do not depend on
- * the teams. Waits at a team barrier are bounded: a team that times out makes fvg_batch_run return FVG_EHIP.
+ * force, every instance of at least 2 min_chunk cells gets min(max_team, N / min_chunk). The chunks of a 2-D
+ * instance are slabs of whole rows, at least 3 rows and min_chunk cells each. Results do not depend on the teams. Waits at a team barrier are bounded: a team that times out makes fvg_batch_run return FVG_EHIP.
  * max_team = 1, first-order faces or a device without cooperative launches: one workgroup per instance.
  * Defaults: max_team 8, min_chunk 512, force 0. max_team <= 16, min_chunk >= 3. */
 int fvg_batch_set_teams(fvg_batch* b, int max_team, int min_chunk, int force);
