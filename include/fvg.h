@@ -312,7 +312,7 @@ int fvg_batch_get_block(const fvg_batch* b, int* block);
  * architecture, instance sizes, scheme) is appended, and a later call for the same key reads it instead of
  * measuring (*from_cache = 1). */
 int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cache_path, int* block, int* from_cache);
-/* Teams of 1-D instances (no reference counterpart: the reference runs one process per instance). The largest
+/* Teams of 1-D and 2-D instances (no reference counterpart: the reference runs one process per instance). The largest
  * instances run on several workgroups of one cooperative launch, each over its own chunk of at least min_chunk
  * cells, with two barriers among a team's workgroups per stage; the other instances are taken one at a time, longest
  * first, by any workgroup. An instance gets a team when its estimated work is at least twice the average per resident
