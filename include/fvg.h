@@ -312,9 +312,9 @@ int fvg_batch_get_block(const fvg_batch* b, int* block);
  * architecture, instance sizes, scheme) is appended, and a later call for the same key reads it instead of
  * measuring (*from_cache = 1). */
 int fvg_batch_tune(fvg_batch* b, double t_probe, int max_steps, const char* cache_path, int* block, int* from_cache);
-/* Teams of 1-D and 2-D instances (no reference counterpart: the reference runs one process per instance). The largest
- * instances run on several workgroups of one cooperative launch, each over its own chunk of at least min_chunk
- * cells, with two barriers among a team's workgroups per stage; the other instances are taken one at a time, longest
+/* Teams of 1-D and 2-D instances (no reference counterpart: the reference runs one process per instance). The
+ * largest instances run on several workgroups of one cooperative launch, each over its own chunk of at least
+ * min_chunk cells, with two barriers among a team's workgroups per stage; the other instances are taken one at a time, longest
  * first, by any workgroup. An instance gets a team when its estimated work is at least twice the average per resident
  * workgroup (as many workgroups as averages, at most max_team; teams use at most half of the workgroups); with
  * force, every instance of at least 2 min_chunk cells gets min(max_team, N / min_chunk). The chunks of a 2-D
