@@ -546,8 +546,8 @@ def main():
         print(json.dumps(side_workload(a.side)), flush=True)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
-              "see DESIGN.md for how the mesh would be split across GPUs", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for {a.gpus} GPUs); the mesh would "
+              "be cut into z-slabs, one per GPU, with halo exchanges between neighbours", file=sys.stderr)
         sys.exit(2)
 
     n = a.n
